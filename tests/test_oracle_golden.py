@@ -1,0 +1,137 @@
+"""Pins the oracle (CPU restatement) against the reference BQCD Fortran.
+
+Golden vectors come from tests/golden/make_golden.py, which runs the reference
+CPU path compiled from source (oracle/ref/Makefile).  The RNG known-answer
+file is the reference's own ranf_test.reference, restated by the sequence of
+calls in ranf_test.F90:25-80.
+"""
+import os
+import re
+
+import numpy as np
+import pytest
+
+import oracle_lib as ol
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+TOL = 1e-13  # relative to the field's max magnitude, per north-star parity bar
+
+
+def rel(a, b):
+    return float(np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300))
+
+
+def setup_case(f):
+    lat = ol.Lattice(tuple(int(x) for x in f["L"]))
+    seed = int(f["seed"])
+    u = lat.gauge_exp_random(seed, float(f["scale"]))
+    eta_e = lat.spinor_gauss(seed + 1, 0)
+    eta_o = lat.spinor_gauss(seed + 2, 1)
+    mom = lat.momenta_gauss(seed + 3)
+    kappa, csw, rho = float(f["kappa"]), float(f["csw"]), float(f["rho"])
+    return lat, u, eta_e, eta_o, mom, kappa, csw, rho
+
+
+def test_ranf_known_answers():
+    """ranf/ranset/ranget against ranf_test.reference (integers exact)."""
+    text = open(os.path.join(HERE, "golden", "ranf_test.reference")).read()
+    blocks = re.split(r"\n\s*\n\s*\n", text)
+    rows = [tuple(line.split()) for line in text.splitlines() if re.match(r"^\s+-?\d+\s+\d+\s+[\d.]+$", line)]
+    assert len(rows) == 10 + 10 + 31 + 31 + 48 + 48
+    got = []
+    ol.ranset(0, 0)
+    for i in range(1, 11):
+        s = ol.ranget(); got.append((i, s, ol.ranf()))
+    ol.ranset(4711, 0)
+    for i in range(1, 11):
+        s = ol.ranget(); got.append((i, s, ol.ranf()))
+    for i in range(-10, 21):
+        ol.ranset(i, 0); s = ol.ranget(); got.append((i, s, ol.ranf()))
+    for i in range(-10, 21):
+        ol.ranset(0, i); s = ol.ranget(); got.append((i, s, ol.ranf()))
+    for i in range(48):
+        ol.ranset(2 ** i, 0); s = ol.ranget(); got.append((i, s, ol.ranf()))
+    for i in range(48):
+        ol.ranset(0, 2 ** i); s = ol.ranget(); got.append((i, s, ol.ranf()))
+    for (ri, rs, rx), (gi, gs, gx) in zip(rows, got):
+        assert int(ri) == gi and int(rs) == gs
+        assert abs(float(rx) - gx) <= 1e-15
+    assert len(blocks) > 1
+
+
+def test_generator_inputs_bit_exact(golden):
+    f = golden("l4444")
+    lat, u, eta_e, eta_o, mom, *_ = setup_case(f)
+    assert np.array_equal(u, f["u"])
+    assert np.array_equal(eta_e, f["eta_e"])
+    assert np.array_equal(eta_o, f["eta_o"])
+    assert np.array_equal(mom, f["mom"])
+
+
+@pytest.mark.parametrize("case", ["l4444", "l4846"])
+def test_oracle_full_fields(golden, case):
+    f = golden(case)
+    lat, u, eta_e, eta_o, mom, kappa, csw, rho = setup_case(f)
+    if "u_checksum" in f:
+        assert np.allclose(f["u_checksum"], [np.sum(u), np.sum(u * u)], rtol=1e-14)
+    a, ai, b = lat.clover_init(u, csw * kappa)
+    assert rel(a, f["clover_a"]) < TOL
+    assert rel(ai, f["clover_i"]) < TOL
+    assert rel(b, f["clover_b"]) < TOL
+    uf = lat.flip_bc(u)
+    assert rel(lat.d(0, 0, eta_o, uf), f["d_eo"]) < TOL
+    assert rel(lat.d(1, 0, eta_e, uf), f["d_oe"]) < TOL
+    assert rel(lat.d(0, 1, eta_o, uf), f["ddag_eo"]) < TOL
+    assert rel(lat.d(1, 1, eta_e, uf), f["ddag_oe"]) < TOL
+    assert rel(lat.clover_mult_a(a[0], eta_e), f["cma_e"]) < TOL
+    assert rel(lat.clover_mult_a(a[1], eta_o), f["cma_o"]) < TOL
+    assert rel(lat.clover_mult_a(ai[1], eta_o), f["cmao_o"]) < TOL
+    assert rel(lat.clover_mult_a(ai[0], eta_e), f["cmao_e"]) < TOL
+    assert rel(lat.clover_mult_b(b[1], eta_o), f["cmb_o"]) < TOL
+    op = lat.op(uf, a, ai, kappa, csw * kappa, rho)
+    for name, which in (("mtil", 0), ("mtil_dag", 1), ("mtdagmt", 2), ("w_mult", 3), ("wdw", 5)):
+        assert rel(lat.apply(op, which, eta_e), f[name]) < TOL, name
+    x, it, rc = lat.cg(op, 2, eta_e, float(f["rest"]), int(f["maxiter"]))
+    assert rc == 0 and it == int(f["info_cg_iters"])
+    assert rel(x, f["cg_x"]) < 1e-12
+    x, it, rc = lat.cg(op, 5, eta_e, float(f["rest"]), int(f["maxiter"]), x0=0.01 * eta_o)
+    assert rc == 0 and it == int(f["info_cgw_iters"])
+    assert rel(x, f["cgw_x"]) < 1e-12
+    bvec = lat.apply(op, 0, eta_e)
+    step = float(f["step"])
+    assert rel(lat.dsf(op, uf, step, eta_e, bvec, b), f["dsf"]) < TOL
+    assert rel(lat.dsd(b, step, csw * kappa, u), f["dsd"]) < TOL
+    assert rel(lat.dsg(u, step, float(f["beta"])), f["dsg"]) < TOL
+    assert abs(lat.clover_action(b[1]) - float(f["info_clover_action"])) < 1e-12 * abs(
+        float(f["info_clover_action"]))
+    if "u_new" in f:
+        un = lat.hmc_u(u, mom, step)
+        assert rel(un, f["u_new"]) < TOL
+        a2, _, _ = lat.clover_init(un, csw * kappa)
+        assert rel(a2, f["clover_a_new"]) < TOL
+
+
+def test_oracle_sampled_8888(golden):
+    f = golden("l8888")
+    lat, u, eta_e, eta_o, mom, kappa, csw, rho = setup_case(f)
+    assert np.allclose(f["u_checksum"], [np.sum(u), np.sum(u * u)], rtol=1e-14)
+    sites = f["sites"]
+    a, ai, b = lat.clover_init(u, csw * kappa)
+    uf = lat.flip_bc(u)
+    op = lat.op(uf, a, ai, kappa, csw * kappa, rho)
+    got = {
+        "d_eo": lat.d(0, 0, eta_o, uf),
+        "ddag_oe": lat.d(1, 1, eta_e, uf),
+        "mtdagmt": lat.apply(op, 2, eta_e),
+        "wdw": lat.apply(op, 5, eta_e),
+    }
+    for k, v in got.items():
+        assert rel(v[sites], f[k + "_sample"]) < TOL, k
+        assert abs(np.sum(v * v) - f[k + "_norm2"]) < 1e-12 * f[k + "_norm2"]
+    x, it, rc = lat.cg(op, 2, eta_e, float(f["rest"]), int(f["maxiter"]))
+    assert it == int(f["info_cg_iters"])
+    assert rel(x[sites], f["cg_x_sample"]) < 1e-12
+    p = lat.dsf(op, uf, float(f["step"]), eta_e, lat.apply(op, 0, eta_e), b)
+    assert rel(p[:, :, sites], f["dsf_sample"]) < TOL
+    p = lat.dsd(b, float(f["step"]), csw * kappa, u)
+    assert rel(p[:, :, sites], f["dsd_sample"]) < TOL
