@@ -1,0 +1,235 @@
+"""Python host side of the gfx950 BQCD hot-path library (ctypes over the C ABI
+declared in include/bqcd_hip.h).
+
+The names mirror the reference's Fortran interface: d / d_dag
+(d/D21.F90:29), clover_mult_a / clover_mult_ao / clover_mult_b
+(clover/clover_mult_*.F90), mtil / mtil_dag (m_tilde.F90), mtdagmt
+(mtdagmt.F90), w_mult / w_mult_dag / w_dagger_w (w_mult.F90), cg (cg.F90:51)
+and the forces dsf / dsd / dsg.  Host arrays are numpy float64 in the
+reference's layouts:
+  spinor (volh, 3, 4, 2); gauge (4, 2, stride, 3, 3, 2); clover
+  (2, volh, 2, 18, 2); momenta (4, 2, stride, 8).
+Device ("_d") calls take torch tensors in the library's SoA layout:
+  spinor (12, volh, 2) float64.
+
+There is no CPU fallback: if libbqcd_hip.so is missing or a call fails, an
+exception is raised.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libbqcd_hip.so")
+
+OP = {"mtil": 0, "mtil_dag": 1, "mtdagmt": 2, "w_mult": 3, "w_mult_dag": 4, "w_dagger_w": 5}
+CLOVER = {"a": 0, "ainv": 1, "b": 2}
+PREC = {"f64": 64, "f32": 32, "f16": 16}
+
+_dp = ctypes.POINTER(ctypes.c_double)
+_ip = ctypes.POINTER(ctypes.c_int)
+_vp = ctypes.c_void_p
+
+
+class HipError(RuntimeError):
+    pass
+
+
+class NoConvergence(HipError):
+    pass
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise HipError("libbqcd_hip.so not built (run __graft_entry__.build()): %s" % LIB_PATH)
+        L = ctypes.CDLL(LIB_PATH)
+        L.hb_last_error.restype = ctypes.c_char_p
+        L.hb_init.argtypes = [ctypes.c_int * 4, ctypes.c_int * 4, ctypes.c_int, ctypes.POINTER(_vp)]
+        L.hb_finalize.argtypes = [_vp]
+        L.hb_volh.argtypes = [_vp]
+        L.hb_load_gauge.argtypes = [_vp, _dp, ctypes.c_int, ctypes.c_int]
+        L.hb_save_gauge.argtypes = [_vp, _dp, ctypes.c_int, ctypes.c_int]
+        L.hb_load_clover.argtypes = [_vp, _dp, _dp, _dp]
+        L.hb_save_clover.argtypes = [_vp, _dp, _dp, _dp]
+        L.hb_dslash.argtypes = [_vp, ctypes.c_int, ctypes.c_int, _dp, _dp]
+        L.hb_clover_mult.argtypes = [_vp, ctypes.c_int, ctypes.c_int, _dp, _dp]
+        L.hb_apply.argtypes = [_vp, ctypes.c_int, ctypes.c_double, ctypes.c_double, _dp, _dp]
+        L.hb_cg.argtypes = [_vp, ctypes.c_int, ctypes.c_double, ctypes.c_double, _dp, _dp,
+                            ctypes.c_float, ctypes.c_int, ctypes.c_int, _ip, _dp]
+        L.hb_set_stream.argtypes = [_vp, _vp]
+        L.hb_dslash_d.argtypes = [_vp, ctypes.c_int, ctypes.c_int, _vp, _vp]
+        L.hb_apply_d.argtypes = [_vp, ctypes.c_int, ctypes.c_double, ctypes.c_double, _vp, _vp]
+        L.hb_cg_d.argtypes = [_vp, ctypes.c_int, ctypes.c_double, ctypes.c_double, _vp, _vp,
+                              ctypes.c_float, ctypes.c_int, ctypes.c_int, _ip, _dp]
+        L.hb_spinor_to_device.argtypes = [_vp, _vp, _dp]
+        L.hb_spinor_to_host.argtypes = [_vp, _dp, _vp]
+        L.hb_last_timing.argtypes = [_vp, _dp, _ip]
+        _lib = L
+    return _lib
+
+
+def _check(rc):
+    if rc != 0:
+        msg = lib().hb_last_error().decode(errors="replace")
+        if rc == -4:
+            raise NoConvergence(msg)
+        raise HipError("bqcd_hip error %d: %s" % (rc, msg))
+
+
+def _p(a):
+    if a is None:
+        return None
+    assert a.dtype == np.float64 and a.flags["C_CONTIGUOUS"], "need C-contiguous float64"
+    return a.ctypes.data_as(_dp)
+
+
+def _tp(t):
+    """device pointer of a torch tensor"""
+    return ctypes.c_void_p(t.data_ptr())
+
+
+class Lattice:
+    """One lattice on one GPU: geometry, links and clover term resident in HBM."""
+
+    def __init__(self, L, bc=(1, 1, 1, -1), device=0):
+        self.L = tuple(int(x) for x in L)
+        self.bc = tuple(int(x) for x in bc)
+        self._h = _vp()
+        _check(lib().hb_init((ctypes.c_int * 4)(*self.L), (ctypes.c_int * 4)(*self.bc), device,
+                             ctypes.byref(self._h)))
+        self.volh = lib().hb_volh(self._h)
+
+    def close(self):
+        if self._h:
+            lib().hb_finalize(self._h)
+            self._h = _vp()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # ---- host allocation helpers ------------------------------------------
+    def spinor(self):
+        return np.zeros((self.volh, 3, 4, 2))
+
+    # ---- fields -------------------------------------------------------------
+    def load_gauge(self, u, flipped=False):
+        u = np.ascontiguousarray(u, dtype=np.float64)
+        stride = u.shape[2]
+        _check(lib().hb_load_gauge(self._h, _p(u), stride, 1 if flipped else 0))
+
+    def save_gauge(self, flipped=False, stride=None):
+        stride = stride or self.volh
+        u = np.zeros((4, 2, stride, 3, 3, 2))
+        _check(lib().hb_save_gauge(self._h, _p(u), stride, 1 if flipped else 0))
+        return u
+
+    def load_clover(self, a=None, ainv=None, b=None):
+        f = [None if x is None else np.ascontiguousarray(x, dtype=np.float64) for x in (a, ainv, b)]
+        _check(lib().hb_load_clover(self._h, *[_p(x) for x in f]))
+
+    def save_clover(self):
+        a, ai, b = (np.zeros((2, self.volh, 2, 18, 2)) for _ in range(3))
+        _check(lib().hb_save_clover(self._h, _p(a), _p(ai), _p(b)))
+        return a, ai, b
+
+    # ---- operators (host arrays) --------------------------------------------
+    def d(self, e, inp, dagger=False):
+        out = self.spinor()
+        _check(lib().hb_dslash(self._h, e, 1 if dagger else 0, _p(out),
+                               _p(np.ascontiguousarray(inp, dtype=np.float64))))
+        return out
+
+    def d_dag(self, e, inp):
+        return self.d(e, inp, dagger=True)
+
+    def clover_mult(self, which, parity, inp):
+        out = self.spinor()
+        _check(lib().hb_clover_mult(self._h, CLOVER[which], parity, _p(out),
+                                    _p(np.ascontiguousarray(inp, dtype=np.float64))))
+        return out
+
+    def clover_mult_a(self, parity, inp):
+        return self.clover_mult("a", parity, inp)
+
+    def clover_mult_ao(self, parity, inp):
+        return self.clover_mult("ainv", parity, inp)
+
+    def clover_mult_b(self, parity, inp):
+        return self.clover_mult("b", parity, inp)
+
+    def apply(self, op, inp, kappa, rho=0.0):
+        out = self.spinor()
+        _check(lib().hb_apply(self._h, OP[op], kappa, rho, _p(out),
+                              _p(np.ascontiguousarray(inp, dtype=np.float64))))
+        return out
+
+    def mtil(self, inp, kappa):
+        return self.apply("mtil", inp, kappa)
+
+    def mtil_dag(self, inp, kappa):
+        return self.apply("mtil_dag", inp, kappa)
+
+    def mtdagmt(self, inp, kappa):
+        return self.apply("mtdagmt", inp, kappa)
+
+    def w_mult(self, inp, kappa, rho):
+        return self.apply("w_mult", inp, kappa, rho)
+
+    def w_mult_dag(self, inp, kappa, rho):
+        return self.apply("w_mult_dag", inp, kappa, rho)
+
+    def w_dagger_w(self, inp, kappa, rho):
+        return self.apply("w_dagger_w", inp, kappa, rho)
+
+    # ---- solver -------------------------------------------------------------
+    def cg(self, op, b, kappa, rho=0.0, rest=1e-20, maxiter=1000, log=0, x0=None):
+        """cg(matrix_mult, x, b, ...) of cg.F90:51: returns (x, iterations, true |r|^2)."""
+        x = self.spinor() if x0 is None else np.array(x0, dtype=np.float64, copy=True)
+        it = ctypes.c_int(0)
+        tr = ctypes.c_double(0.0)
+        _check(lib().hb_cg(self._h, OP[op], kappa, rho, _p(x),
+                           _p(np.ascontiguousarray(b, dtype=np.float64)), ctypes.c_float(rest),
+                           maxiter, log, ctypes.byref(it), ctypes.byref(tr)))
+        return x, it.value, tr.value
+
+    # ---- device-resident API (torch tensors, SoA layout) ---------------------
+    def set_stream(self, stream=None):
+        _check(lib().hb_set_stream(self._h, ctypes.c_void_p(stream.cuda_stream) if stream else None))
+
+    def to_device(self, host_spinor, out):
+        _check(lib().hb_spinor_to_device(self._h, _tp(out),
+                                         _p(np.ascontiguousarray(host_spinor, dtype=np.float64))))
+        return out
+
+    def to_host(self, t):
+        h = self.spinor()
+        _check(lib().hb_spinor_to_host(self._h, _p(h), _tp(t)))
+        return h
+
+    def dslash_d(self, e, dagger, out, inp):
+        _check(lib().hb_dslash_d(self._h, e, 1 if dagger else 0, _tp(out), _tp(inp)))
+
+    def apply_d(self, op, kappa, rho, out, inp):
+        _check(lib().hb_apply_d(self._h, OP[op], kappa, rho, _tp(out), _tp(inp)))
+
+    def cg_d(self, op, x, b, kappa, rho=0.0, rest=1e-20, maxiter=1000, log=0):
+        it = ctypes.c_int(0)
+        tr = ctypes.c_double(0.0)
+        _check(lib().hb_cg_d(self._h, OP[op], kappa, rho, _tp(x), _tp(b), ctypes.c_float(rest),
+                             maxiter, log, ctypes.byref(it), ctypes.byref(tr)))
+        return it.value, tr.value
+
+    def last_timing(self):
+        ms = ctypes.c_double(0.0)
+        n = ctypes.c_int(0)
+        _check(lib().hb_last_timing(self._h, ctypes.byref(ms), ctypes.byref(n)))
+        return ms.value, n.value

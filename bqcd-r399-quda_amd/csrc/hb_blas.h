@@ -1,0 +1,36 @@
+// hb_blas.h - spinor vector kernels, reductions and CG device state.
+#pragma once
+
+#include "hb_kernels.h"
+
+namespace hb {
+
+// device-resident CG scalars (cg.F90:61-116); kernels read and update them so
+// that an iteration needs no host round trip
+struct CgState {
+  double rtr_old, rtr, pap, beta;
+  double rest;     // stopping bound on |r|^2 (cg_para%rest, promoted to FP64)
+  int iter, maxiter;
+  int done;        // 0 running, 1 converged, 2 maxiter reached
+  int pad;
+  double res[4];   // scratch results (true residual etc.)
+};
+
+int reduce_grid(const Context *c, int n);
+int clover_mult_f64(Context *c, int which, int eo, double2 *out, const double2 *in);
+int cg_start(Context *c, double2 *r, double2 *p, const double2 *b, CgState *st);
+int cg_update(Context *c, double2 *x, double2 *r, const double2 *p, const double2 *q, CgState *st);
+int cg_pupdate(Context *c, double2 *p, const double2 *r, const CgState *st);
+int residual(Context *c, double2 *out, const double2 *b, double *res_d);
+int dot_norm(Context *c, const double2 *x, const double2 *y, double *res_d);
+int zero_f64(Context *c, double2 *p, size_t n);
+int copy_f64(Context *c, double2 *d, const double2 *s, size_t n);
+
+// host helpers from hb_fields.hip
+int spinor_upload(Context *c, double2 *dst_d, const double *host);
+int spinor_download(Context *c, double *host, const double2 *src_d);
+int gen_upload(Context *c, double *dst_d, const double *host, int stride);
+int gen_download(Context *c, double *host, const double *src_d, int stride);
+int alloc_fp64_fields(Context *c);
+
+}  // namespace hb

@@ -1,0 +1,183 @@
+// hb_dslash.hip - fused even-odd clover-Wilson hopping matrix.
+//
+// One thread per output site.  Each thread gathers the 8 neighbour spinors
+// (coalesced: neighbours of consecutive sites are consecutive in the SoA
+// planes), projects them to half spinors in registers, multiplies by the 8
+// links and reconstructs -- the whole of d/D21.F90:29-76 (projection, the
+// four direction passes and the out read-modify-write) in a single pass that
+// reads each operand once and writes the result once.  Periodic wrap and the
+// antiperiodic sign are computed on the device from the site coordinates.
+//
+// Epilogues fuse the operator algebra of m_tilde.F90:43-114 into the write:
+//   EPI_PLAIN  out = D in
+//   EPI_CLINV  out = A^-1 (D in)                      (clover_mult_ao on conf%i)
+//   EPI_MTIL   out = A x0 - kappa^2 (D in) + rho x0   (clover_mult_a + sc_xpby + sc_axpy)
+//   EPI_MTDOT  EPI_MTIL and a grid reduction of Re<v, out> (sc_dot), single pass
+#include "hb_dslash.h"
+
+namespace hb {
+
+// One hop: acc += (1 + s gamma_mu) U psi(x +- mu), U = U_mu(x) forward or
+// U_mu(x - mu)^+ backward, with the boundary sign when the hop wraps.
+template <class S, int MU, bool FWD, int SG>
+__device__ __forceinline__ void hop(typename S::V (&acc)[12], const DslashArgs<S> &a,
+                                    const SpinorView<S> &in, const Geom &g, int e, int i,
+                                    const int (&xc)[4]) {
+  using V = typename S::V;
+  using A = typename S::A;
+  const int vh = g.vh;
+  int c[4] = {xc[0], xc[1], xc[2], xc[3]};
+  A sgn = (A)1;
+  if (FWD) {
+    c[MU] += 1;
+    if (c[MU] == g.L[MU]) { c[MU] = 0; sgn = (A)g.bc[MU]; }
+  } else {
+    c[MU] -= 1;
+    if (c[MU] < 0) { c[MU] = g.L[MU] - 1; sgn = (A)g.bc[MU]; }
+  }
+  const int j = site_index(g, c[0], c[1], c[2], c[3]);
+  V h[6], w[6];
+  if constexpr (MU == 3) {
+    V p[6];
+    in.load_pair(p, j, SG < 0 ? 2 : 0);
+    const A f = (A)2 * sgn;
+#pragma unroll
+    for (int cc = 0; cc < 3; cc++) {
+      h[2 * cc] = cscale<V>(f, p[cc]);
+      h[2 * cc + 1] = cscale<V>(f, p[3 + cc]);
+    }
+  } else {
+    V p[12];
+    in.load(p, j);
+    project<MU, SG>(h, p);
+    if (sgn < (A)0) {
+#pragma unroll
+      for (int k = 0; k < 6; k++) h[k] = cmk<V>(-h[k].x, -h[k].y);
+    }
+  }
+  V u[9];
+  if (FWD) {
+    load_link<S>(u, a.U, MU, e, i, vh);
+    su3_mul_half(w, u, h);
+  } else {
+    load_link<S>(u, a.U, MU, 1 - e, j, vh);
+    su3_adj_mul_half(w, u, h);
+  }
+  reconstruct<MU, SG>(acc, w);
+}
+
+// Completes one hop before the next starts: the accumulator is pinned in
+// registers (so its arithmetic cannot be sunk to the final store) and the
+// scheduler may not hoist the next hop's loads above this point.
+template <class V> __device__ __forceinline__ void pin(V (&acc)[12]) {
+#pragma unroll
+  for (int k = 0; k < 12; k++) asm volatile("" : "+v"(acc[k].x), "+v"(acc[k].y));
+}
+#define HB_SCHED_FENCE() do { pin(acc); __builtin_amdgcn_sched_barrier(0); } while (0)
+
+template <class S, bool DAG, int EPI>
+__global__ __launch_bounds__(kBlock) void k_dslash(DslashArgs<S> a) {
+  using V = typename S::V;
+  using A = typename S::A;
+  const Geom g = a.g;
+  const int vh = g.vh;
+  if (a.skip && *a.skip) return;
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  V acc[12];
+#pragma unroll
+  for (int k = 0; k < 12; k++) acc[k] = cmk<V>((A)0, (A)0);
+  const int e = a.out_parity;
+  const bool active = i < vh;
+  if (active) {
+    int xc[4];
+    site_coords(g, e, i, xc[0], xc[1], xc[2], xc[3]);
+    const SpinorView<S> in(a.in, vh);
+    // forward projector sign: (1 - gamma) for D, (1 + gamma) for D^+
+    constexpr int SF = DAG ? 1 : -1;
+    constexpr int SB = -SF;
+    hop<S, 3, true, SF>(acc, a, in, g, e, i, xc);  HB_SCHED_FENCE();
+    hop<S, 3, false, SB>(acc, a, in, g, e, i, xc); HB_SCHED_FENCE();
+    hop<S, 2, true, SF>(acc, a, in, g, e, i, xc);  HB_SCHED_FENCE();
+    hop<S, 2, false, SB>(acc, a, in, g, e, i, xc); HB_SCHED_FENCE();
+    hop<S, 1, true, SF>(acc, a, in, g, e, i, xc);  HB_SCHED_FENCE();
+    hop<S, 1, false, SB>(acc, a, in, g, e, i, xc); HB_SCHED_FENCE();
+    hop<S, 0, true, SF>(acc, a, in, g, e, i, xc);  HB_SCHED_FENCE();
+    hop<S, 0, false, SB>(acc, a, in, g, e, i, xc); HB_SCHED_FENCE();
+  }
+
+  if constexpr (EPI == EPI_PLAIN) {
+    if (active) store_spinor<S>(a.out, i, vh, acc);
+  } else if constexpr (EPI == EPI_CLINV) {
+    if (active) {
+      if (a.C) {
+        V r[12];
+        clover_apply<S, 0>(r, acc, a.C, a.Cn, e, i, vh);
+        store_spinor<S>(a.out, i, vh, r);
+      } else {
+        store_spinor<S>(a.out, i, vh, acc);
+      }
+    }
+  } else {
+    double dot = 0.0;
+    if (active) {
+      V x0[12], r[12];
+      load_spinor<S>(x0, a.x0, i, vh);
+      // the hop sum is complete; keep it from being interleaved with the
+      // clover loads
+      HB_SCHED_FENCE();
+      if (a.C) clover_apply<S, 0>(r, x0, a.C, a.Cn, e, i, vh);
+      else {
+#pragma unroll
+        for (int k = 0; k < 12; k++) r[k] = x0[k];
+      }
+      const A mk2 = (A)a.mkappa2, rho = (A)a.rho;
+#pragma unroll
+      for (int k = 0; k < 12; k++) {
+        r[k].x += mk2 * acc[k].x + rho * x0[k].x;
+        r[k].y += mk2 * acc[k].y + rho * x0[k].y;
+      }
+      store_spinor<S>(a.out, i, vh, r);
+      if constexpr (EPI == EPI_MTDOT) {
+        V v[12];
+        load_spinor<S>(v, a.dotv, i, vh);
+#pragma unroll
+        for (int k = 0; k < 12; k++) dot += (double)v[k].x * (double)r[k].x + (double)v[k].y * (double)r[k].y;
+      }
+    }
+    if constexpr (EPI == EPI_MTDOT) {
+      double vals[1] = {dot};
+      double *res = a.dot_out;
+      grid_reduce<1>(vals, a.partials, a.counter, [res](const double *s) { res[0] = s[0]; });
+    }
+  }
+}
+
+template <class S>
+int launch_dslash(const DslashArgs<S> &a, bool dag, int epi, hipStream_t st) {
+  dim3 grid(nblocks(a.g.vh)), block(kBlock);
+#define HB_DS(D, E) hipLaunchKernelGGL((k_dslash<S, D, E>), grid, block, 0, st, a)
+  if (!dag) {
+    switch (epi) {
+      case EPI_PLAIN: HB_DS(false, EPI_PLAIN); break;
+      case EPI_CLINV: HB_DS(false, EPI_CLINV); break;
+      case EPI_MTIL: HB_DS(false, EPI_MTIL); break;
+      default: HB_DS(false, EPI_MTDOT); break;
+    }
+  } else {
+    switch (epi) {
+      case EPI_PLAIN: HB_DS(true, EPI_PLAIN); break;
+      case EPI_CLINV: HB_DS(true, EPI_CLINV); break;
+      case EPI_MTIL: HB_DS(true, EPI_MTIL); break;
+      default: HB_DS(true, EPI_MTDOT); break;
+    }
+  }
+#undef HB_DS
+  HB_HIP(hipGetLastError());
+  return HB_OK;
+}
+
+template int launch_dslash<SF64>(const DslashArgs<SF64> &, bool, int, hipStream_t);
+template int launch_dslash<SF32>(const DslashArgs<SF32> &, bool, int, hipStream_t);
+template int launch_dslash<SF16>(const DslashArgs<SF16> &, bool, int, hipStream_t);
+
+}  // namespace hb

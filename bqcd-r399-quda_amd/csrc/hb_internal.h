@@ -1,0 +1,248 @@
+// hb_internal.h - shared definitions of the gfx950 BQCD hot-path library.
+//
+// Device layouts (site-major structure of arrays; `vh` = sites per parity):
+//   spinor  T2 s[12][vh]            component k = 3*spin + colour
+//   gauge   T2 U[4 mu][2 eo][9][vh] element k = 3*row + col; physical links
+//                                   (the fermion boundary sign is applied in
+//                                   the kernels, never stored)
+//   clover  T2 C[2 eo][36][vh]      per parity: block J words 18*J + w, in the
+//                                   reference's packed word order
+//                                   (type_clover_a / type_clover_b)
+//   momenta double P[4][2][8][vh]
+// Site index within a parity: i = (x + LX*(y + LY*(z + LZ*t))) / 2, parity
+// (x+y+z+t) & 1, the reference's even-odd order (init_common.F90:110-205).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <hip/hip_fp16.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <string>
+#include <vector>
+
+#include "../../include/bqcd_hip.h"
+
+namespace hb {
+
+// ----------------------------------------------------------------------------
+// errors
+// ----------------------------------------------------------------------------
+void set_error(const char *fmt, ...);
+
+#define HB_HIP(call)                                                             \
+  do {                                                                           \
+    hipError_t e_ = (call);                                                      \
+    if (e_ != hipSuccess) {                                                      \
+      ::hb::set_error("%s:%d: %s: %s", __FILE__, __LINE__, #call,                \
+                      hipGetErrorString(e_));                                    \
+      return HB_E_HIP;                                                           \
+    }                                                                            \
+  } while (0)
+
+#define HB_CHECK(cond, code, ...)                                                \
+  do {                                                                           \
+    if (!(cond)) {                                                               \
+      ::hb::set_error(__VA_ARGS__);                                              \
+      return (code);                                                             \
+    }                                                                            \
+  } while (0)
+
+#define HB_TRY(expr)                                                             \
+  do {                                                                           \
+    int r_ = (expr);                                                             \
+    if (r_ != HB_OK) return r_;                                                  \
+  } while (0)
+
+// ----------------------------------------------------------------------------
+// geometry passed by value to every kernel
+// ----------------------------------------------------------------------------
+struct Geom {
+  int L[4];     // LX LY LZ LT
+  int nxh;      // LX/2
+  int vh;       // sites per parity
+  int bc[4];    // fermion boundary sign per direction (+1/-1)
+};
+
+// coordinates of site i of parity eo
+__device__ __forceinline__ void site_coords(const Geom &g, int eo, int i, int &x, int &y, int &z,
+                                            int &t) {
+  int xh = i % g.nxh;
+  int r = i / g.nxh;
+  y = r % g.L[1];
+  r /= g.L[1];
+  z = r % g.L[2];
+  t = r / g.L[2];
+  x = 2 * xh + ((y + z + t + eo) & 1);
+}
+
+__device__ __forceinline__ int site_index(const Geom &g, int x, int y, int z, int t) {
+  return (x >> 1) + g.nxh * (y + g.L[1] * (z + g.L[2] * t));
+}
+
+// ----------------------------------------------------------------------------
+// context
+// ----------------------------------------------------------------------------
+struct Field {
+  void *p = nullptr;
+  size_t bytes = 0;
+};
+
+struct Scalars;  // device CG state, see hb_cg.hip
+
+struct Context {
+  Geom g;
+  int device = 0;
+  hipStream_t stream = nullptr;      // stream in use
+  hipStream_t own_stream = nullptr;  // created by hb_init
+  // FP64 fields
+  double2 *gauge = nullptr;          // [4][2][9][vh]
+  double2 *clov[3] = {nullptr, nullptr, nullptr};  // A/2, A^-1/2, LDL^+ : [2][36][vh]
+  bool have_gauge = false, have_clover[3] = {false, false, false};
+  // low-precision copies (built lazily from the FP64 fields)
+  float2 *gauge32 = nullptr;
+  float2 *clov32[2] = {nullptr, nullptr};          // A, A^-1
+  __half2 *gauge16 = nullptr;
+  __half2 *clov16[2] = {nullptr, nullptr};
+  float *clov16n[2] = {nullptr, nullptr};          // per-site per-block scale
+  uint64_t gauge_version = 1, clover_version = 1;
+  uint64_t lp_gauge_version = 0, lp_clover_version = 0;
+  // scratch
+  std::vector<Field> scratch;
+  double *red = nullptr;             // reduction partials
+  unsigned int *red_count = nullptr; // last-block counters
+  int red_blocks = 0;
+  void *host_stage = nullptr;        // pinned staging
+  size_t host_stage_bytes = 0;
+  void *dev_stage = nullptr;         // device staging (AoS copies)
+  size_t dev_stage_bytes = 0;
+  double last_solve_ms = 0.0;
+  int last_n_matvec = 0;
+};
+
+int ensure_dev_stage(Context *c, size_t bytes);
+int ensure_host_stage(Context *c, size_t bytes);
+void *scratch(Context *c, int slot, size_t bytes, int *rc);
+
+constexpr int kBlock = 256;
+inline int nblocks(int n, int b = kBlock) { return (n + b - 1) / b; }
+
+// ----------------------------------------------------------------------------
+// complex helpers
+// ----------------------------------------------------------------------------
+template <typename T> struct C2;
+template <> struct C2<double> { using t = double2; };
+template <> struct C2<float> { using t = float2; };
+
+template <typename V> __device__ __forceinline__ V cmk(decltype(V{}.x) a, decltype(V{}.x) b) {
+  V r; r.x = a; r.y = b; return r;
+}
+template <typename V> __device__ __forceinline__ V cadd(V a, V b) { return cmk<V>(a.x + b.x, a.y + b.y); }
+template <typename V> __device__ __forceinline__ V csub(V a, V b) { return cmk<V>(a.x - b.x, a.y - b.y); }
+template <typename V> __device__ __forceinline__ V cmul(V a, V b) {
+  return cmk<V>(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+// a^* b
+template <typename V> __device__ __forceinline__ V cmulc(V a, V b) {
+  return cmk<V>(a.x * b.x + a.y * b.y, a.x * b.y - a.y * b.x);
+}
+template <typename V> __device__ __forceinline__ V cconj(V a) { return cmk<V>(a.x, -a.y); }
+template <typename V> __device__ __forceinline__ V ci(V a) { return cmk<V>(-a.y, a.x); }       // i a
+template <typename V> __device__ __forceinline__ V cmi(V a) { return cmk<V>(a.y, -a.x); }      // -i a
+template <typename V, typename S> __device__ __forceinline__ V cscale(S s, V a) {
+  return cmk<V>(s * a.x, s * a.y);
+}
+// acc += a * b
+template <typename V> __device__ __forceinline__ void cfma(V &acc, V a, V b) {
+  acc.x += a.x * b.x - a.y * b.y;
+  acc.y += a.x * b.y + a.y * b.x;
+}
+// acc += conj(a) * b
+template <typename V> __device__ __forceinline__ void cfmac(V &acc, V a, V b) {
+  acc.x += a.x * b.x + a.y * b.y;
+  acc.y += a.x * b.y - a.y * b.x;
+}
+
+// ----------------------------------------------------------------------------
+// wavefront / block reductions (64-wide wavefronts)
+// ----------------------------------------------------------------------------
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+// Sums v over the block; result valid in thread 0.  `lds` needs blockDim/64 doubles.
+__device__ __forceinline__ double block_sum(double v, double *lds) {
+  v = wave_sum(v);
+  int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  if (lane == 0) lds[w] = v;
+  __syncthreads();
+  double s = 0.0;
+  if (threadIdx.x == 0) {
+    int nw = (blockDim.x + 63) >> 6;
+    for (int k = 0; k < nw; k++) s += lds[k];
+  }
+  return s;
+}
+
+// Single-pass deterministic grid reduction of NV values: every block writes its
+// partials, the last block to finish sums them in block order and calls fin(sums)
+// from thread 0.  counter must start at 0; the last block resets it.
+template <int NV, typename Fin>
+__device__ __forceinline__ void grid_reduce(double (&v)[NV], double *partials,
+                                            unsigned int *counter, Fin fin) {
+  __shared__ double lds[NV][16];
+  __shared__ bool last;
+#pragma unroll
+  for (int k = 0; k < NV; k++) {
+    double s = wave_sum(v[k]);
+    if ((threadIdx.x & 63) == 0) lds[k][threadIdx.x >> 6] = s;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int nw = (blockDim.x + 63) >> 6;
+#pragma unroll
+    for (int k = 0; k < NV; k++) {
+      double s = 0.0;
+      for (int w = 0; w < nw; w++) s += lds[k][w];
+      partials[(size_t)k * gridDim.x + blockIdx.x] = s;
+    }
+    __threadfence();
+    unsigned int prev = atomicAdd(counter, 1u);
+    last = (prev == gridDim.x - 1);
+  }
+  __syncthreads();
+  if (!last) return;
+  __threadfence();
+  // the last block sums all partials (in fixed order per thread, then tree)
+  double acc[NV];
+#pragma unroll
+  for (int k = 0; k < NV; k++) {
+    double s = 0.0;
+    for (int b = threadIdx.x; b < (int)gridDim.x; b += blockDim.x)
+      s += partials[(size_t)k * gridDim.x + b];
+    acc[k] = s;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < NV; k++) {
+    double s = wave_sum(acc[k]);
+    if ((threadIdx.x & 63) == 0) lds[k][threadIdx.x >> 6] = s;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int nw = (blockDim.x + 63) >> 6;
+    double tot[NV];
+#pragma unroll
+    for (int k = 0; k < NV; k++) {
+      double s = 0.0;
+      for (int w = 0; w < nw; w++) s += lds[k][w];
+      tot[k] = s;
+    }
+    *counter = 0u;
+    fin(tot);
+  }
+}
+
+}  // namespace hb

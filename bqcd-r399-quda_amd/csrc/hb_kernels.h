@@ -1,0 +1,358 @@
+// hb_kernels.h - device building blocks shared by the hopping-matrix, clover,
+// solver and force kernels: storage policies (FP64 / FP32 / FP16 with
+// per-site norms), spinor and link access, spin projection, SU(3) products,
+// and the packed 6x6 clover block products.
+#pragma once
+
+#include "hb_internal.h"
+
+namespace hb {
+
+// ----------------------------------------------------------------------------
+// raw buffer access: a buffer resource (base, byte size) in SGPRs plus a
+// per-thread 32-bit byte offset in one VGPR and a wave-uniform plane offset in
+// an SGPR -- all planes of a site share one offset register.
+// ----------------------------------------------------------------------------
+using rsrc_t = __amdgpu_buffer_rsrc_t;
+typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ rsrc_t make_rsrc(const void *p, uint32_t bytes) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(p), (short)0, (int)bytes, 0x00020000);
+}
+__device__ __forceinline__ double2 bld_f64x2(rsrc_t r, uint32_t vo, uint32_t so) {
+  return __builtin_bit_cast(double2, __builtin_amdgcn_raw_buffer_load_b128(r, vo, so, 0));
+}
+__device__ __forceinline__ float2 bld_f32x2(rsrc_t r, uint32_t vo, uint32_t so) {
+  return __builtin_bit_cast(float2, __builtin_amdgcn_raw_buffer_load_b64(r, vo, so, 0));
+}
+__device__ __forceinline__ __half2 bld_f16x2(rsrc_t r, uint32_t vo, uint32_t so) {
+  return __builtin_bit_cast(__half2, __builtin_amdgcn_raw_buffer_load_b32(r, vo, so, 0));
+}
+__device__ __forceinline__ float bld_f32(rsrc_t r, uint32_t vo, uint32_t so) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, vo, so, 0));
+}
+
+// ----------------------------------------------------------------------------
+// storage policies: A = arithmetic type, V = complex of A, ST = stored complex
+// ----------------------------------------------------------------------------
+struct SF64 {
+  using A = double;
+  using V = double2;
+  using ST = double2;
+  static constexpr bool kNorm = false;
+  static constexpr int kBytes = 16;
+  __device__ __forceinline__ static V ld(const ST *p, size_t k) { return p[k]; }
+  __device__ __forceinline__ static void st(ST *p, size_t k, V v) { p[k] = v; }
+  __device__ __forceinline__ static V bld(rsrc_t r, uint32_t vo, uint32_t so) { return bld_f64x2(r, vo, so); }
+};
+
+struct SF32 {
+  using A = float;
+  using V = float2;
+  using ST = float2;
+  static constexpr bool kNorm = false;
+  static constexpr int kBytes = 8;
+  __device__ __forceinline__ static V ld(const ST *p, size_t k) { return p[k]; }
+  __device__ __forceinline__ static void st(ST *p, size_t k, V v) { p[k] = v; }
+  __device__ __forceinline__ static V bld(rsrc_t r, uint32_t vo, uint32_t so) { return bld_f32x2(r, vo, so); }
+};
+
+// FP16 storage, FP32 arithmetic; spinors and clover blocks carry a per-site
+// FP32 scale (the maximum magnitude of the site's components)
+struct SF16 {
+  using A = float;
+  using V = float2;
+  using ST = __half2;
+  static constexpr bool kNorm = true;
+  static constexpr int kBytes = 4;
+  __device__ __forceinline__ static V ld(const ST *p, size_t k) { return __half22float2(p[k]); }
+  __device__ __forceinline__ static void st(ST *p, size_t k, V v) { p[k] = __float22half2_rn(v); }
+  __device__ __forceinline__ static V bld(rsrc_t r, uint32_t vo, uint32_t so) {
+    return __half22float2(bld_f16x2(r, vo, so));
+  }
+};
+
+// a spinor field of one parity: 12 planes of vh complex numbers (+ norms)
+template <class S> struct SpinorPtr {
+  typename S::ST *p;
+  float *norm;  // FP16 only: one scale per site
+};
+
+// buffer view of a spinor field, built once per kernel (wave-uniform)
+template <class S> struct SpinorView {
+  rsrc_t r, rn;
+  uint32_t plane;  // bytes per plane
+  __device__ __forceinline__ SpinorView(const SpinorPtr<S> &f, int vh) {
+    plane = (uint32_t)vh * S::kBytes;
+    r = make_rsrc(f.p, plane * 12u);
+    if constexpr (S::kNorm) rn = make_rsrc(f.norm, (uint32_t)vh * 4u);
+  }
+  __device__ __forceinline__ void load(typename S::V (&v)[12], int i) const {
+    const uint32_t vo = (uint32_t)i * S::kBytes;
+#pragma unroll
+    for (int k = 0; k < 12; k++) v[k] = S::bld(r, vo, plane * k);
+    if constexpr (S::kNorm) {
+      float n = bld_f32(rn, (uint32_t)i * 4u, 0);
+#pragma unroll
+      for (int k = 0; k < 12; k++) { v[k].x *= n; v[k].y *= n; }
+    }
+  }
+  // spin components s0, s0+1 (k = 3*s0 .. 3*s0+5) -- the t projector
+  __device__ __forceinline__ void load_pair(typename S::V (&v)[6], int i, int s0) const {
+    const uint32_t vo = (uint32_t)i * S::kBytes;
+#pragma unroll
+    for (int k = 0; k < 6; k++) v[k] = S::bld(r, vo, plane * (3 * s0 + k));
+    if constexpr (S::kNorm) {
+      float n = bld_f32(rn, (uint32_t)i * 4u, 0);
+#pragma unroll
+      for (int k = 0; k < 6; k++) { v[k].x *= n; v[k].y *= n; }
+    }
+  }
+};
+
+template <class S>
+__device__ __forceinline__ void load_spinor(typename S::V (&v)[12], const SpinorPtr<S> &f, int i,
+                                            int vh) {
+#pragma unroll
+  for (int k = 0; k < 12; k++) v[k] = S::ld(f.p, (size_t)k * vh + i);
+  if constexpr (S::kNorm) {
+    float n = f.norm[i];
+#pragma unroll
+    for (int k = 0; k < 12; k++) { v[k].x *= n; v[k].y *= n; }
+  }
+}
+
+template <class S, class VIn>
+__device__ __forceinline__ void store_spinor(const SpinorPtr<S> &f, int i, int vh,
+                                             const VIn (&v)[12]) {
+  using V = typename S::V;
+  using A = typename S::A;
+  if constexpr (S::kNorm) {
+    float m = 0.f;
+#pragma unroll
+    for (int k = 0; k < 12; k++) m = fmaxf(m, fmaxf(fabsf((float)v[k].x), fabsf((float)v[k].y)));
+    float inv = m > 0.f ? 1.0f / m : 0.f;
+    f.norm[i] = m;
+#pragma unroll
+    for (int k = 0; k < 12; k++)
+      S::st(f.p, (size_t)k * vh + i, cmk<V>((A)v[k].x * inv, (A)v[k].y * inv));
+  } else {
+#pragma unroll
+    for (int k = 0; k < 12; k++) S::st(f.p, (size_t)k * vh + i, cmk<V>((A)v[k].x, (A)v[k].y));
+  }
+}
+
+// ----------------------------------------------------------------------------
+// links: U[4][2][9][vh], element 3*row + col; one buffer per (mu, eo) slab
+// ----------------------------------------------------------------------------
+template <class S>
+__device__ __forceinline__ void load_link(typename S::V (&u)[9], const typename S::ST *U, int mu,
+                                          int eo, int i, int vh) {
+  const uint32_t plane = (uint32_t)vh * S::kBytes;
+  rsrc_t r = make_rsrc(U + (size_t)((mu * 2 + eo) * 9) * vh, plane * 9u);
+  const uint32_t vo = (uint32_t)i * S::kBytes;
+#pragma unroll
+  for (int k = 0; k < 9; k++) u[k] = S::bld(r, vo, plane * k);
+}
+
+// w = U h  (h: 3 colours x 2 spins, index 2*c + s)
+template <class V>
+__device__ __forceinline__ void su3_mul_half(V (&w)[6], const V (&u)[9], const V (&h)[6]) {
+#pragma unroll
+  for (int r = 0; r < 3; r++) {
+    V a0 = cmul(u[3 * r], h[0]);
+    V a1 = cmul(u[3 * r], h[1]);
+    cfma(a0, u[3 * r + 1], h[2]);
+    cfma(a1, u[3 * r + 1], h[3]);
+    cfma(a0, u[3 * r + 2], h[4]);
+    cfma(a1, u[3 * r + 2], h[5]);
+    w[2 * r] = a0;
+    w[2 * r + 1] = a1;
+  }
+}
+
+// w = U^+ h
+template <class V>
+__device__ __forceinline__ void su3_adj_mul_half(V (&w)[6], const V (&u)[9], const V (&h)[6]) {
+#pragma unroll
+  for (int r = 0; r < 3; r++) {
+    V a0 = cmulc(u[r], h[0]);
+    V a1 = cmulc(u[r], h[1]);
+    cfmac(a0, u[3 + r], h[2]);
+    cfmac(a1, u[3 + r], h[3]);
+    cfmac(a0, u[6 + r], h[4]);
+    cfmac(a1, u[6 + r], h[5]);
+    w[2 * r] = a0;
+    w[2 * r + 1] = a1;
+  }
+}
+
+// ----------------------------------------------------------------------------
+// spin projection (1 + s*gamma_mu) and reconstruction, for the gamma basis of
+// the reference (d/D21.F90:79-182, d/D21xyzt.F90:27-78):
+//   gamma_k = [[0, G_k], [G_k^+, 0]], G_x = [[0,i],[i,0]], G_y = [[0,1],[-1,0]],
+//   G_z = [[i,0],[0,-i]];  gamma_t = diag(1,1,-1,-1).
+// Half spinor h = psi_up + s*G psi_lo; lower half of the result = s*G^+ (U h).
+// Spinor index k = 3*spin + colour; half spinor index 2*colour + spin.
+// ----------------------------------------------------------------------------
+template <int MU, int SG, class V>
+__device__ __forceinline__ void project(V (&h)[6], const V (&p)[12]) {
+#pragma unroll
+  for (int c = 0; c < 3; c++) {
+    V p0 = p[c], p1 = p[3 + c], p2 = p[6 + c], p3 = p[9 + c];
+    if constexpr (MU == 0) {  // h0 = p0 + s i p3, h1 = p1 + s i p2
+      h[2 * c] = SG > 0 ? cadd(p0, ci(p3)) : csub(p0, ci(p3));
+      h[2 * c + 1] = SG > 0 ? cadd(p1, ci(p2)) : csub(p1, ci(p2));
+    } else if constexpr (MU == 1) {  // h0 = p0 + s p3, h1 = p1 - s p2
+      h[2 * c] = SG > 0 ? cadd(p0, p3) : csub(p0, p3);
+      h[2 * c + 1] = SG > 0 ? csub(p1, p2) : cadd(p1, p2);
+    } else {  // MU == 2: h0 = p0 + s i p2, h1 = p1 - s i p3
+      h[2 * c] = SG > 0 ? cadd(p0, ci(p2)) : csub(p0, ci(p2));
+      h[2 * c + 1] = SG > 0 ? csub(p1, ci(p3)) : cadd(p1, ci(p3));
+    }
+  }
+}
+
+template <int MU, int SG, class V>
+__device__ __forceinline__ void reconstruct(V (&acc)[12], const V (&w)[6]) {
+#pragma unroll
+  for (int c = 0; c < 3; c++) {
+    V w0 = w[2 * c], w1 = w[2 * c + 1];
+    if constexpr (MU == 3) {
+      if constexpr (SG < 0) {  // (1 - gamma_t): spins 3,4
+        acc[6 + c] = cadd(acc[6 + c], w0);
+        acc[9 + c] = cadd(acc[9 + c], w1);
+      } else {
+        acc[c] = cadd(acc[c], w0);
+        acc[3 + c] = cadd(acc[3 + c], w1);
+      }
+    } else {
+      acc[c] = cadd(acc[c], w0);
+      acc[3 + c] = cadd(acc[3 + c], w1);
+      if constexpr (MU == 0) {  // lo0 = -s i w1, lo1 = -s i w0
+        acc[6 + c] = SG > 0 ? csub(acc[6 + c], ci(w1)) : cadd(acc[6 + c], ci(w1));
+        acc[9 + c] = SG > 0 ? csub(acc[9 + c], ci(w0)) : cadd(acc[9 + c], ci(w0));
+      } else if constexpr (MU == 1) {  // lo0 = -s w1, lo1 = s w0
+        acc[6 + c] = SG > 0 ? csub(acc[6 + c], w1) : cadd(acc[6 + c], w1);
+        acc[9 + c] = SG > 0 ? cadd(acc[9 + c], w0) : csub(acc[9 + c], w0);
+      } else {  // MU == 2: lo0 = -s i w0, lo1 = s i w1
+        acc[6 + c] = SG > 0 ? csub(acc[6 + c], ci(w0)) : cadd(acc[6 + c], ci(w0));
+        acc[9 + c] = SG > 0 ? cadd(acc[9 + c], ci(w1)) : csub(acc[9 + c], ci(w1));
+      }
+    }
+  }
+}
+
+// ----------------------------------------------------------------------------
+// packed Hermitian 6x6 blocks (type_clover_a word order, clover/clover.h):
+//   w0 = (A11, A22) real pair, w1..5 = A12..A16, w6..9 = A23..A26,
+//   w10 = (A33, A44), w11..13 = A34..A36, w14,15 = A45, A46, w16 = (A55, A66),
+//   w17 = A56.
+// LDL^+ factors (type_clover_b): w[r(r-1)/2 + c] = L_rc (r > c),
+//   w15..17 = (D1, D2), (D3, D4), (D5, D6) with D_k the stored 1/2 * 1/d_k.
+// ----------------------------------------------------------------------------
+__host__ __device__ constexpr int a_off_word(int r, int c) {  // r < c
+  return r == 0 ? c : r == 1 ? 4 + c : r == 2 ? 8 + c : r == 3 ? 10 + c : 17;
+}
+
+// y = A x for one block; w = the block's 18 words
+template <class V>
+__device__ __forceinline__ void herm6(V (&y)[6], const V (&w)[18], const V (&x)[6]) {
+  using A = decltype(V{}.x);
+  A d[6] = {w[0].x, w[0].y, w[10].x, w[10].y, w[16].x, w[16].y};
+#pragma unroll
+  for (int r = 0; r < 6; r++) y[r] = cscale<V>(d[r], x[r]);
+#pragma unroll
+  for (int r = 0; r < 6; r++)
+#pragma unroll
+    for (int c = r + 1; c < 6; c++) {
+      V a = w[a_off_word(r, c)];
+      cfma(y[r], a, x[c]);   // A_rc x_c
+      cfmac(y[c], a, x[r]);  // A_cr x_r = conj(A_rc) x_r
+    }
+}
+
+// x = (LDL^+)^-1 y for one block (forward then back substitution)
+template <class V>
+__device__ __forceinline__ void ldl6(V (&x)[6], const V (&w)[18], const V (&yin)[6]) {
+  using A = decltype(V{}.x);
+  V y[6];
+#pragma unroll
+  for (int r = 0; r < 6; r++) {
+    V acc = yin[r];
+#pragma unroll
+    for (int c = 0; c < r; c++) {
+      V l = w[r * (r - 1) / 2 + c];
+      acc = csub(acc, cmul(l, y[c]));
+    }
+    y[r] = acc;
+  }
+  A d[6] = {w[15].x, w[15].y, w[16].x, w[16].y, w[17].x, w[17].y};
+#pragma unroll
+  for (int r = 5; r >= 0; r--) {
+    V acc = cscale<V>(d[r], y[r]);
+#pragma unroll
+    for (int c = r + 1; c < 6; c++) {
+      V l = w[c * (c - 1) / 2 + r];
+      acc = csub(acc, cmulc(l, x[c]));
+    }
+    x[r] = acc;
+  }
+}
+
+// loads block J of the clover field of parity eo at site i
+template <class S>
+__device__ __forceinline__ void load_clover_block(typename S::V (&w)[18], const typename S::ST *C,
+                                                  const float *Cn, int eo, int J, int i, int vh) {
+  const uint32_t plane = (uint32_t)vh * S::kBytes;
+  rsrc_t r = make_rsrc(C + (size_t)(eo * 36 + J * 18) * vh, plane * 18u);
+  const uint32_t vo = (uint32_t)i * S::kBytes;
+#pragma unroll
+  for (int k = 0; k < 18; k++) w[k] = S::bld(r, vo, plane * k);
+  if constexpr (S::kNorm) {
+    float n = Cn[(size_t)(eo * 2 + J) * vh + i];
+#pragma unroll
+    for (int k = 0; k < 18; k++) { w[k].x *= n; w[k].y *= n; }
+  }
+}
+
+// out = A in (Dirac basis) with A stored as two chiral blocks of A/2
+// (clover_mult_a.F90:44-95); mode 0 = Hermitian product, 1 = LDL^+ solve
+template <class S, int MODE, class V>
+__device__ __forceinline__ void clover_apply(V (&out)[12], const V (&in)[12],
+                                             const typename S::ST *C, const float *Cn, int eo,
+                                             int i, int vh) {
+  V xp[6], y1[6], y2[6];
+#pragma unroll
+  for (int m = 0; m < 6; m++) xp[m] = cadd(in[m], in[m + 6]);
+  {
+    typename S::V w[18];
+    load_clover_block<S>(w, C, Cn, eo, 0, i, vh);
+    V ww[18];
+#pragma unroll
+    for (int k = 0; k < 18; k++) ww[k] = cmk<V>(w[k].x, w[k].y);
+    if constexpr (MODE == 0) herm6(y1, ww, xp); else ldl6(y1, ww, xp);
+  }
+  // finish block 0 before loading block 1 (bounds live registers)
+#pragma unroll
+  for (int m = 0; m < 6; m++) asm volatile("" : "+v"(y1[m].x), "+v"(y1[m].y));
+  __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+  for (int m = 0; m < 6; m++) xp[m] = csub(in[m], in[m + 6]);
+  {
+    typename S::V w[18];
+    load_clover_block<S>(w, C, Cn, eo, 1, i, vh);
+    V ww[18];
+#pragma unroll
+    for (int k = 0; k < 18; k++) ww[k] = cmk<V>(w[k].x, w[k].y);
+    if constexpr (MODE == 0) herm6(y2, ww, xp); else ldl6(y2, ww, xp);
+  }
+#pragma unroll
+  for (int m = 0; m < 6; m++) {
+    out[m] = cadd(y1[m], y2[m]);
+    out[m + 6] = csub(y1[m], y2[m]);
+  }
+}
+
+}  // namespace hb

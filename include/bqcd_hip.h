@@ -1,0 +1,174 @@
+/*
+ * bqcd_hip.h - C ABI of the MI355X (gfx950) implementation of the BQCD
+ * fermion hot path: even-odd preconditioned clover-Wilson hopping matrix,
+ * clover multiplies/inverse, the CG solve on M~^+M~ (or W^+W), and the
+ * fermion / clover-determinant / gauge forces.
+ *
+ * Plain C types only.  Every entry point returns 0 on success or a negative
+ * HB_E_* code; hb_last_error() gives a message.  A Fortran iso_c_binding
+ * module (bqcd-r399-quda_amd/fortran/bqcd_hip_solver.F90) maps these onto
+ * BQCD's cg() / quda_solver() interfaces.
+ *
+ * Host-side field layouts are BQCD's own (Fortran column-major):
+ *   spinor   complex(4 spin, 3 colour, volh)                SPINCOL_FIELD
+ *   gauge    complex(3, 3, site_stride, 0:1, 4)             GAUGE_FIELD
+ *   clover   type_clover_a/b (2 blocks, volh, 0:1), 18 complex words/block
+ *   momenta  real(8, site_stride, 0:1, 4)                   GENERATOR_FIELD
+ * site_stride is volh_tot for arrays allocated by BQCD (only the first volh
+ * sites of each (eo, mu) slab are read/written) or volh for packed arrays.
+ *
+ * Device-side ("_d") entry points take device pointers in the library's own
+ * site-major SoA layout: a spinor is 12 planes of volh complex numbers,
+ * component k = 3*spin + colour, i.e. double[12][volh][2]; momenta are
+ * double[4][2][8][volh].  Parity 0 = EVEN, 1 = ODD, as BQCD.
+ */
+#ifndef BQCD_HIP_H
+#define BQCD_HIP_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct hb_context hb_context;
+
+enum {
+  HB_OK = 0,
+  HB_E_ARG = -1,       /* invalid argument (shape, parity, null pointer) */
+  HB_E_HIP = -2,       /* HIP runtime error */
+  HB_E_STATE = -3,     /* field not loaded / wrong order of calls */
+  HB_E_NOCONV = -4,    /* CG did not converge within maxiter (log != 2) */
+  HB_E_NOMEM = -5
+};
+
+/* operators, as the reference's operator routines */
+enum {
+  HB_OP_MTIL = 0,       /* m_tilde.F90:43      M~ = A_ee - kappa^2 D_eo A_oo^-1 D_oe */
+  HB_OP_MTIL_DAG = 1,   /* m_tilde.F90:80      M~^+                                 */
+  HB_OP_MTDAGMT = 2,    /* mtdagmt.F90:28      M~^+ M~                              */
+  HB_OP_W = 3,          /* w_mult.F90:28       W = M~ + rho                          */
+  HB_OP_W_DAG = 4,      /* w_mult.F90:46       W^+                                   */
+  HB_OP_WDAGW = 5       /* w_mult.F90:64       W^+ W                                 */
+};
+
+/* clover fields */
+enum { HB_CLOVER_A = 0, HB_CLOVER_AINV = 1, HB_CLOVER_B = 2 };
+
+/* precisions of the inner solver */
+enum { HB_PREC_F64 = 64, HB_PREC_F32 = 32, HB_PREC_F16 = 16 };
+
+const char *hb_last_error(void);
+
+/* ---- lifecycle: replaces init_quda / comm_set_gridsize (invert_quda hook,
+ *      bqcd.F90:75-76) and end_quda (bqcd.F90:93) ------------------------- */
+/* L = (LX, LY, LZ, LT) even extents; bc = fermion boundary signs (+1/-1) per
+ * direction (bc_fermions, module_lattice.F90); device = HIP ordinal. */
+int hb_init(const int L[4], const int bc[4], int device, hb_context **ctx);
+int hb_finalize(hb_context *ctx);
+int hb_volh(const hb_context *ctx);
+
+/* ---- fields (host, BQCD layout) ---------------------------------------- */
+/* Uploads conf%u.  links_are_bc_flipped != 0 means the caller has applied
+ * flip_bc (flip_bc.F90:97-121) as BQCD does before every solve; the library
+ * keeps physical links and applies the boundary sign in its kernels. */
+int hb_load_gauge(hb_context *ctx, const double *u, int site_stride, int links_are_bc_flipped);
+int hb_save_gauge(hb_context *ctx, double *u, int site_stride, int bc_flipped);
+/* Uploads conf%a, conf%i, conf%b (any may be NULL). rho_shift is added to
+ * the diagonal of A (folded as +rho/2 on the stored A/2, as clover_rho,
+ * invert_quda.F90:70-93); pass 0 and use the rho argument of the operators
+ * instead when the clover field is shared between W and M~ solves. */
+int hb_load_clover(hb_context *ctx, const double *a, const double *ainv, const double *b);
+/* clover_init (clover/clover_init.F90:28-78) on the device from the loaded
+ * physical links: builds A/2, A^-1/2 and the LDL^+ factors for both parities */
+int hb_build_clover(hb_context *ctx, double csw_kappa);
+int hb_save_clover(hb_context *ctx, double *a, double *ainv, double *b);
+/* clover_action (clover/clover_action.F90:29-62) of the odd-parity B field */
+int hb_clover_action(hb_context *ctx, double *s_det);
+
+/* ---- operators (host spinors, BQCD layout) ------------------------------ */
+/* d / d_dag (d/D21.F90:29-76): out (parity e) = D in (parity 1-e) */
+int hb_dslash(hb_context *ctx, int out_parity, int dagger, double *out, const double *in);
+/* clover_mult_a / clover_mult_ao / clover_mult_b (clover/clover_mult_*.F90)
+ * on one parity: which = HB_CLOVER_A (A), AINV (A^-1 via conf%i),
+ * B (A^-1 via the LDL^+ factors) */
+int hb_clover_mult(hb_context *ctx, int which, int parity, double *out, const double *in);
+/* mtil, mtil_dag, mtdagmt, w_mult, w_mult_dag, w_dagger_w (HB_OP_*) */
+int hb_apply(hb_context *ctx, int op, double kappa, double rho, double *out, const double *in);
+
+/* ---- solver: replaces cg() (cg.F90:51) and quda_solver()
+ *      (invert_quda.F90:96-196), same contract -------------------------- */
+/* Solves op x = b (op = HB_OP_MTDAGMT, or HB_OP_WDAGW with rho).  x is the
+ * initial guess on entry and the solution on exit.  Stops when the iterated
+ * |r|^2 <= rest (absolute, rest is the float32 cg_para%rest).  log == 2
+ * accepts no convergence (returns 0 with iters = maxiter), otherwise
+ * HB_E_NOCONV.  true_rtr (may be NULL) receives |b - op x|^2 recomputed in
+ * FP64, the re-check quda_solver does (invert_quda.F90:155-168). */
+int hb_cg(hb_context *ctx, int op, double kappa, double rho, double *x, const double *b,
+          float rest, int maxiter, int log, int *iters, double *true_rtr);
+
+/* Mixed precision CG with reliable updates (defect correction): inner
+ * iterations in FP32 or FP16 storage (FP32 arithmetic); FP64 true residual,
+ * solution accumulation and global sums.  A reliable update happens when the
+ * iterated residual drops by delta against its maximum since the last update
+ * or after max_inner iterations.  Same stopping rule and outputs as hb_cg;
+ * iters counts inner iterations, n_reliable (may be NULL) the updates. */
+int hb_cg_mixed(hb_context *ctx, int op, double kappa, double rho, double *x, const double *b,
+                float rest, int maxiter, int log, int precision, double delta, int max_inner,
+                int *iters, int *n_reliable, double *true_rtr);
+
+/* Multi right-hand-side solve: nrhs systems (op with kappa[k], rho[k]) with
+ * host spinors x[k], b[k] (BQCD layout, each 24*volh doubles).  Right-hand
+ * sides sharing (kappa, clover) are processed by one block hopping matrix
+ * that reads every link once for the whole block; converged systems are
+ * masked out.  rest[k] as hb_cg. block = RHS block size (0 = autotuned). */
+int hb_cg_multi(hb_context *ctx, int op, int nrhs, const double *kappa, const double *rho,
+                double *const *x, const double *const *b, const float *rest, int maxiter,
+                int log, int precision, int block, int *iters, double *true_rtr);
+
+/* ---- forces (host, BQCD layout; p accumulates) -------------------------- */
+/* dsf (dsf.F90:28-110): hopping + clover part of dS_f/dU for spinors a, b
+ * (even parity).  The links held by the context are physical. */
+int hb_dsf(hb_context *ctx, double *p, int site_stride, double step, double kappa,
+           double csw_kappa, const double *a, const double *b);
+/* dsd (dsd.F90:28-49): clover-determinant force */
+int hb_dsd(hb_context *ctx, double *p, int site_stride, double step, double csw_kappa);
+/* dsg (dsg.F90:28-63): Wilson gauge force */
+int hb_dsg(hb_context *ctx, double *p, int site_stride, double step, double beta);
+/* hmc_u (hmc_u.F90:28-69): U <- exp(i step p.lambda) U, reunitarised; the
+ * clover term is rebuilt when csw_kappa != 0 */
+int hb_hmc_u(hb_context *ctx, const double *p, int site_stride, double step, double csw_kappa);
+/* sum over plaquettes of Re Tr U_p (action.F90:70-123 uses it for S_g) */
+int hb_plaquette(hb_context *ctx, double *re_tr_sum);
+
+/* ---- device-pointer entry points (library SoA layout, device memory) --- */
+/* stream: a hipStream_t (NULL = the context's own stream) */
+int hb_set_stream(hb_context *ctx, void *stream);
+int hb_dslash_d(hb_context *ctx, int out_parity, int dagger, double *out_d, const double *in_d);
+int hb_apply_d(hb_context *ctx, int op, double kappa, double rho, double *out_d,
+               const double *in_d);
+int hb_cg_d(hb_context *ctx, int op, double kappa, double rho, double *x_d, const double *b_d,
+            float rest, int maxiter, int log, int *iters, double *true_rtr);
+int hb_cg_mixed_d(hb_context *ctx, int op, double kappa, double rho, double *x_d,
+                  const double *b_d, float rest, int maxiter, int log, int precision,
+                  double delta, int max_inner, int *iters, int *n_reliable, double *true_rtr);
+int hb_cg_multi_d(hb_context *ctx, int op, int nrhs, const double *kappa, const double *rho,
+                  double *const *x_d, const double *const *b_d, const float *rest, int maxiter,
+                  int log, int precision, int block, int *iters, double *true_rtr);
+/* BQCD layout <-> SoA conversions of one spinor (host <-> device) */
+int hb_spinor_to_device(hb_context *ctx, double *dst_d, const double *src_host);
+int hb_spinor_to_host(hb_context *ctx, double *dst_host, const double *src_d);
+/* dsf / autograd helpers on device spinors; p_d is SoA momenta */
+int hb_dsf_d(hb_context *ctx, double *p_d, double step, double kappa, double csw_kappa,
+             const double *a_d, const double *b_d);
+/* synthetic fields on the device (seeded, deterministic on a given GPU) */
+int hb_gauge_random(hb_context *ctx, unsigned long long seed, double scale);
+int hb_gauge_smear(hb_context *ctx, int sweeps, double alpha);
+int hb_spinor_random_d(hb_context *ctx, double *s_d, unsigned long long seed);
+
+/* timing of the last solve: kernel time (ms) of the hopping-matrix launches
+ * and their count, measured with HIP events on the context stream */
+int hb_last_timing(hb_context *ctx, double *solve_ms, int *n_matvec);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,142 @@
+"""GPU parity of the FP64 hot path against the reference (golden vectors) and
+the oracle (same seeded inputs).  Bar: <= 1e-13 relative per field (max over
+sites of the abs difference / max magnitude), identical CG iteration counts.
+"""
+import numpy as np
+import pytest
+
+import oracle_lib as ol
+
+pytestmark = pytest.mark.gpu
+TOL = 1e-13
+
+
+def rel(a, b):
+    return float(np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300))
+
+
+@pytest.fixture(scope="module")
+def hip():
+    import bqcd_hip
+    return bqcd_hip
+
+
+def make_case(f):
+    lat = ol.Lattice(tuple(int(x) for x in f["L"]))
+    seed = int(f["seed"])
+    u = lat.gauge_exp_random(seed, float(f["scale"]))
+    eta_e = lat.spinor_gauss(seed + 1, 0)
+    eta_o = lat.spinor_gauss(seed + 2, 1)
+    return lat, u, eta_e, eta_o
+
+
+@pytest.mark.parametrize("case", ["l4444", "l4846"])
+def test_dslash_clover_ops_vs_reference(hip, golden, case):
+    f = golden(case)
+    lat, u, eta_e, eta_o = make_case(f)
+    kappa, csw, rho = float(f["kappa"]), float(f["csw"]), float(f["rho"])
+    g = hip.Lattice(lat.L)
+    g.load_gauge(lat.flip_bc(u), flipped=True)   # as BQCD passes conf%u to the solver
+    g.load_clover(f["clover_a"], f["clover_i"], f["clover_b"])
+    assert rel(g.d(0, eta_o), f["d_eo"]) < TOL
+    assert rel(g.d(1, eta_e), f["d_oe"]) < TOL
+    assert rel(g.d_dag(0, eta_o), f["ddag_eo"]) < TOL
+    assert rel(g.d_dag(1, eta_e), f["ddag_oe"]) < TOL
+    assert rel(g.clover_mult_a(0, eta_e), f["cma_e"]) < TOL
+    assert rel(g.clover_mult_a(1, eta_o), f["cma_o"]) < TOL
+    assert rel(g.clover_mult_ao(1, eta_o), f["cmao_o"]) < TOL
+    assert rel(g.clover_mult_ao(0, eta_e), f["cmao_e"]) < TOL
+    assert rel(g.clover_mult_b(1, eta_o), f["cmb_o"]) < TOL
+    assert rel(g.mtil(eta_e, kappa), f["mtil"]) < TOL
+    assert rel(g.mtil_dag(eta_e, kappa), f["mtil_dag"]) < TOL
+    assert rel(g.mtdagmt(eta_e, kappa), f["mtdagmt"]) < TOL
+    assert rel(g.w_mult(eta_e, kappa, rho), f["w_mult"]) < TOL
+    assert rel(g.w_dagger_w(eta_e, kappa, rho), f["wdw"]) < TOL
+    g.close()
+
+
+@pytest.mark.parametrize("case", ["l4444", "l4846"])
+def test_cg_vs_reference(hip, golden, case):
+    f = golden(case)
+    lat, u, eta_e, eta_o = make_case(f)
+    kappa, rho = float(f["kappa"]), float(f["rho"])
+    g = hip.Lattice(lat.L)
+    g.load_gauge(u, flipped=False)
+    g.load_clover(f["clover_a"], f["clover_i"], f["clover_b"])
+    rest, maxiter = float(f["rest"]), int(f["maxiter"])
+    x, it, tr = g.cg("mtdagmt", eta_e, kappa, rest=rest, maxiter=maxiter)
+    assert it == int(f["info_cg_iters"])
+    assert rel(x, f["cg_x"]) < 1e-12
+    assert tr <= 2 * rest
+    x, it, tr = g.cg("w_dagger_w", eta_e, kappa, rho, rest=rest, maxiter=maxiter, x0=0.01 * eta_o)
+    assert it == int(f["info_cgw_iters"])
+    assert rel(x, f["cgw_x"]) < 1e-12
+    g.close()
+
+
+@pytest.mark.parametrize("L", [(4, 4, 4, 8), (8, 4, 6, 4), (6, 6, 6, 6), (8, 8, 8, 16)])
+def test_ops_vs_oracle_shapes(hip, L):
+    """ragged extents, every direction wrapping; antiperiodic time"""
+    lat = ol.Lattice(L)
+    u = lat.gauge_exp_random(99 + sum(L), 0.4)
+    a, ai, b = lat.clover_init(u, 1.9 * 0.1355)
+    uf = lat.flip_bc(u)
+    psi_e = lat.spinor_gauss(7, 0)
+    psi_o = lat.spinor_gauss(8, 1)
+    op = lat.op(uf, a, ai, 0.1355, 1.9 * 0.1355, 0.07)
+    g = hip.Lattice(L)
+    g.load_gauge(u)
+    g.load_clover(a, ai, b)
+    for e, inp in ((0, psi_o), (1, psi_e)):
+        for dag in (0, 1):
+            assert rel(g.d(e, inp, dagger=bool(dag)), lat.d(e, dag, inp, uf)) < TOL
+    for name, which in ol.OPS.items():
+        assert rel(g.apply(name, psi_e, 0.1355, 0.07), lat.apply(op, which, psi_e)) < TOL, name
+    g.close()
+
+
+def test_no_clover_and_periodic_bc(hip):
+    """csw = 0 (no clover term: A = 1) and an all-periodic lattice"""
+    L = (4, 6, 4, 4)
+    lat = ol.Lattice(L)
+    u = lat.gauge_exp_random(5, 0.3)
+    psi = lat.spinor_gauss(6, 0)
+    op = lat.op(u, lat.clover(), lat.clover(), 0.12, 0.0, 0.0)  # csw_kappa = 0: clover unused
+    g = hip.Lattice(L, bc=(1, 1, 1, 1))
+    g.load_gauge(u)
+    assert rel(g.mtdagmt(psi, 0.12), lat.apply(op, 2, psi)) < TOL
+    x, it, tr = g.cg("mtdagmt", psi, 0.12, rest=1e-22, maxiter=500)
+    x0, it0, _ = lat.cg(op, 2, psi, 1e-22, 500)
+    assert it == it0 and rel(x, x0) < 1e-12
+    g.close()
+
+
+def test_gauge_roundtrip_and_padded_stride(hip):
+    """BQCD arrays are allocated over volh_tot sites; only the first volh are used"""
+    L = (4, 4, 4, 4)
+    lat = ol.Lattice(L)
+    u = lat.gauge_exp_random(11, 0.3)
+    vh = lat.volh
+    pad = np.full((4, 2, vh + 37, 3, 3, 2), 7.0)
+    pad[:, :, :vh] = lat.flip_bc(u)
+    g = hip.Lattice(L)
+    g.load_gauge(pad, flipped=True)
+    assert np.array_equal(g.save_gauge(flipped=False), u)
+    assert np.array_equal(g.save_gauge(flipped=True), lat.flip_bc(u))
+    g.close()
+
+
+def test_cg_maxiter_and_log(hip, golden):
+    f = golden("l4444")
+    lat, u, eta_e, eta_o = make_case(f)
+    g = hip.Lattice(lat.L)
+    g.load_gauge(u)
+    g.load_clover(f["clover_a"], f["clover_i"], f["clover_b"])
+    with pytest.raises(hip.NoConvergence):
+        g.cg("mtdagmt", eta_e, float(f["kappa"]), rest=1e-30, maxiter=5)
+    x, it, tr = g.cg("mtdagmt", eta_e, float(f["kappa"]), rest=1e-30, maxiter=5, log=2)
+    assert it == 5
+    op = lat.op(lat.flip_bc(u), f["clover_a"], f["clover_i"], float(f["kappa"]), float(f["csw"]) * float(f["kappa"]))
+    x0, it0, rc = lat.cg(op, 2, eta_e, 1e-30, 5)
+    assert rc == 1 and it0 == 5 and rel(x, x0) < 1e-12
+    g.close()
