@@ -1,0 +1,319 @@
+"""Benchmark: CG time to |r|/|b| <= 1e-10 on one MI355X, and FP64 hopping-matrix
+HBM bandwidth.
+
+Workload (the precision configuration): 32^3 x 64 lattice, Haar-random SU(3)
+links smoothed by 5 APE-like sweeps (alpha = 0.36 in the (1-alpha) U + alpha/6 sum
+of staples convention: plaquette ~0.55), kappa = 0.1355, csw = 1.9,
+one unit complex Gaussian source on the even sites, antiperiodic in time.
+A step is one solve of M~^+ M~ x = phi from x = 0 to |r|^2 <= 1e-20 |phi|^2
+(the reference's absolute float32 bound, cg.F90:111) with every field resident
+in HBM.  The solver is selectable; its FP64 true residual is always checked.
+
+Also measured: the FP64 hopping matrix (d, d/D21.F90) in isolation, timed with
+HIP events on the stream it is launched on; algorithmic traffic 1536 B/site
+(8 links x 144 B + in 192 B + out 192 B).
+
+CPU baseline: the reference BQCD Fortran CG (oracle/_ref, built from the
+reference sources with OpenMP) on the same field, a bounded number of
+iterations, scaled to the GPU solve's iteration count; when that binary is
+absent, the oracle's C restatement is timed instead.
+"""
+import argparse
+import json
+import os
+import subprocess
+import sys
+import tempfile
+import time
+
+REPO = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(REPO, "bqcd-r399-quda_amd"))
+
+DSLASH_BYTES_PER_SITE = 8 * 144 + 192 + 192
+DSLASH_FLOPS_PER_SITE = 1320
+HBM_PEAK_GBS = 8000.0
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--lattice", default="32,32,32,64")
+    ap.add_argument("--kappa", type=float, default=0.1355)
+    ap.add_argument("--csw", type=float, default=1.9)
+    ap.add_argument("--solver", default="mixed16", choices=["f64", "mixed32", "mixed16"])
+    ap.add_argument("--delta", type=float, default=0.1)
+    ap.add_argument("--max-inner", type=int, default=1000)
+    ap.add_argument("--dslash-reps", type=int, default=50)
+    ap.add_argument("--cpu-iters", type=int, default=20)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--seed", type=int, default=20261202)
+    ap.add_argument("--sweeps", type=int, default=5)
+    ap.add_argument("--alpha", type=float, default=0.36)
+    ap.add_argument("--only-dslash", type=int, default=0,
+                    help="run only N FP64 hopping-matrix launches and a calibration copy of the "
+                         "same byte count (for hardware-counter runs), then exit")
+    ap.add_argument("--scan", action="store_true",
+                    help="print plaquette and FP64 iteration counts per smearing sweep, then exit")
+    return ap.parse_args()
+
+
+def log(*a):
+    print("[bench]", *a, file=sys.stderr, flush=True)
+
+
+def cpu_baseline(g, b_dev, kappa, csw, gpu_iters, cpu_iters):
+    """Times the reference Fortran CG (or the oracle) on the same field."""
+    import numpy as np
+
+    ref = os.path.join(REPO, "oracle", "_ref", "bqcd_golden_omp")
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or os.cpu_count()
+    L = g.L
+    if os.path.exists(ref):
+        with tempfile.TemporaryDirectory(dir="/tmp") as d:
+            u = g.save_gauge(flipped=False)
+            u.tofile(os.path.join(d, "u.bin"))
+            del u
+            eta = g.to_host(b_dev)
+            eta.tofile(os.path.join(d, "eta_e.bin"))
+            np.zeros_like(eta).tofile(os.path.join(d, "eta_o.bin"))
+            ctl = "%d %d %d %d\n%.17g %.17g 0\n1e-30 %d\n0 0\n%s\n%s\n%d\n" % (
+                *L, kappa, csw, cpu_iters, d, d, cpu_iters)
+            env = dict(os.environ, OMP_NUM_THREADS=str(threads))
+            t0 = time.time()
+            p = subprocess.run([ref], input=ctl.encode(), capture_output=True, env=env, timeout=600)
+            wall = time.time() - t0
+            out = p.stdout.decode()
+            if p.returncode != 0:
+                raise RuntimeError("reference CPU run failed: %s %s" % (out, p.stderr.decode()))
+            tok = out.split()
+            cg_s = float(tok[tok.index("cg_seconds") + 1])
+            it = int(tok[tok.index("cg_iters") + 1])
+        per_iter = cg_s / it
+        return {
+            "value": per_iter * gpu_iters,
+            "unit": "s",
+            "cores": threads,
+            "kind": "reference",
+            "sample": "reference BQCD Fortran cg() (libd21, single_pe, OpenMP, amdflang -O2) "
+                      "on the same 32^3x64 field: %d iterations in %.2f s (%.1f ms/iter, "
+                      "process wall %.1f s incl. clover_init and I/O), scaled to the %d "
+                      "iterations of the GPU FP64 solve" % (it, cg_s, 1e3 * per_iter, wall, gpu_iters),
+        }
+    # oracle port (single thread)
+    sys.path.insert(0, os.path.join(REPO, "tests"))
+    import oracle_lib as ol
+
+    lat = ol.Lattice(L)
+    u = g.save_gauge(flipped=False)
+    a, ai, bcl = lat.clover_init(u, csw * kappa)
+    op = lat.op(lat.flip_bc(u), a, ai, kappa, csw * kappa)
+    eta = g.to_host(b_dev)
+    t0 = time.time()
+    lat.cg(op, 2, eta, 1e-30, cpu_iters)
+    dt = time.time() - t0
+    return {"value": dt / cpu_iters * gpu_iters, "unit": "s", "cores": 1, "kind": "port",
+            "sample": "oracle C restatement cg(), %d iterations in %.2f s, scaled to %d iterations"
+                      % (cpu_iters, dt, gpu_iters)}
+
+
+def scan(g, args, torch, dev):
+    """plaquette and FP64 CG iterations after each smearing sweep"""
+    gen = torch.Generator(device=dev)
+    gen.manual_seed(1)
+    b = torch.randn((12, g.volh, 2), dtype=torch.float64, device=dev, generator=gen)
+    x = torch.zeros_like(b)
+    g.gauge_random(args.seed, 0.0)
+    for sw in range(0, 9):
+        if sw:
+            g.gauge_smear(1, args.alpha)
+        g.build_clover(args.csw * args.kappa)
+        x.zero_()
+        it, tr = g.cg_d("mtdagmt", x, b, args.kappa, rest=1e-20 * float((b * b).sum()),
+                        maxiter=3000, log=2)
+        log("sweeps %d alpha %.2f plaquette %.5f  FP64 CG iters %d  true rel %.2e"
+            % (sw, args.alpha, g.plaquette(), it, tr / float((b * b).sum())))
+    return 0
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build runs on a single MI355X "
+              "(multi-GPU sharding is a design, not part of this build); use --gpus 1",
+              file=sys.stderr)
+        return 2
+    import torch
+    import bqcd_hip
+
+    L = tuple(int(x) for x in args.lattice.split(","))
+    kappa, csw = args.kappa, args.csw
+    dev = torch.device("cuda:0")
+    torch.cuda.set_device(dev)
+    # one explicit stream (default priority) for torch and the library, so that
+    # torch's events time exactly the library's launches
+    stream = torch.cuda.Stream(device=dev)
+    torch.cuda.set_stream(stream)
+
+    g = bqcd_hip.Lattice(L, bc=(1, 1, 1, -1), device=0)
+    g.set_stream(stream)
+    vh = g.volh
+    log("lattice", L, "volh", vh)
+    g.gauge_random(args.seed, 0.0)
+    g.gauge_smear(args.sweeps, args.alpha)
+    plaq = g.plaquette()
+    if args.scan:
+        return scan(g, args, torch, dev)
+    g.build_clover(csw * kappa)
+    log("plaquette <ReTr U_p>/3 = %.6f" % plaq)
+
+    gen = torch.Generator(device=dev)
+    gen.manual_seed(args.seed + 1)
+    b = torch.randn((12, vh, 2), dtype=torch.float64, device=dev, generator=gen) * (0.5 ** 0.5)
+    x = torch.zeros_like(b)
+    bnorm2 = float((b * b).sum())
+    rest = 1e-20 * bnorm2
+
+    # ---- FP64 hopping matrix in isolation -----------------------------------
+    out = torch.empty_like(b)
+    inp = torch.randn((12, vh, 2), dtype=torch.float64, device=dev, generator=gen)
+    for _ in range(5):
+        g.dslash_d(0, False, out, inp)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    for _ in range(args.dslash_reps):
+        g.dslash_d(0, False, out, inp)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    ds_ms = e0.elapsed_time(e1) / args.dslash_reps
+    ds_bytes = DSLASH_BYTES_PER_SITE * vh
+    ds_gbs = ds_bytes / (ds_ms * 1e-3) / 1e9
+    log("FP64 dslash: %.4f ms/launch, %.1f GB/s algorithmic, %.1f GFlop/s"
+        % (ds_ms, ds_gbs, DSLASH_FLOPS_PER_SITE * vh / (ds_ms * 1e-3) / 1e9))
+    if args.only_dslash:
+        for _ in range(args.only_dslash):
+            g.dslash_d(0, False, out, inp)
+        # calibration: a plain device copy moving exactly ds_bytes (read + write)
+        src = torch.empty(ds_bytes // 16, dtype=torch.float64, device=dev)
+        dst = torch.empty_like(src)
+        for _ in range(args.only_dslash):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        log("calibration copy: %d bytes read + %d bytes written per launch" % (ds_bytes // 2, ds_bytes // 2))
+        return 0
+    del out, inp
+
+    # ---- solves ---------------------------------------------------------------
+    def solve_f64():
+        x.zero_()
+        it, tr = g.cg_d("mtdagmt", x, b, kappa, rest=rest, maxiter=20000)
+        return it, 0, tr
+
+    def solve_mixed():
+        x.zero_()
+        return g.cg_mixed_d("mtdagmt", x, b, kappa, rest=rest, maxiter=40000,
+                            precision="f16" if args.solver == "mixed16" else "f32",
+                            delta=args.delta, max_inner=args.max_inner)
+
+    solve = solve_f64 if args.solver == "f64" else solve_mixed
+
+    # FP64 reference solve: time to solution of the plain FP64 GPU path, and
+    # the solution the mixed solver is compared with
+    fp64 = {}
+    if args.solver != "f64":
+        solve_f64()
+        torch.cuda.synchronize()
+        t0 = torch.cuda.Event(enable_timing=True); t1 = torch.cuda.Event(enable_timing=True)
+        t0.record(stream)
+        it64, _, tr64 = solve_f64()
+        t1.record(stream)
+        torch.cuda.synchronize()
+        fp64 = {"ms": t0.elapsed_time(t1), "iters": it64, "true_rel": tr64 / bnorm2, "x": x.clone()}
+        log("FP64 solve: %.1f ms, %d iterations, true |r|^2/|b|^2 = %.3e"
+            % (fp64["ms"], it64, tr64 / bnorm2))
+
+    for _ in range(args.warmup):
+        it, nrel, tr = solve()
+        log("warmup solve: %d iterations, %d reliable updates, true |r|^2/|b|^2 = %.3e"
+            % (it, nrel, tr / bnorm2))
+    torch.cuda.synchronize()
+    times, iters, trs, nrels = [], [], [], []
+    for _ in range(args.steps):
+        torch.cuda.synchronize()
+        s0, s1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        s0.record(stream)
+        it, nrel, tr = solve()
+        s1.record(stream)
+        torch.cuda.synchronize()
+        times.append(s0.elapsed_time(s1))
+        iters.append(it)
+        trs.append(tr)
+        nrels.append(nrel)
+    ms = sum(times) / len(times)
+    true_rel = max(trs) / bnorm2
+    sol_err = None
+    if fp64:
+        sol_err = float(torch.linalg.vector_norm(x - fp64["x"]) / torch.linalg.vector_norm(fp64["x"]))
+    log("%s solve: %.1f ms, %d iterations, %d reliable updates, true |r|^2/|b|^2 = %.3e, "
+        "|x-x64|/|x64| = %s" % (args.solver, ms, iters[0], nrels[0], true_rel, sol_err))
+    if true_rel > 1e-20 * 1.01:
+        log("WARNING: true residual above 1e-20 |b|^2")
+
+    cpu = None
+    if not args.no_cpu:
+        try:
+            cpu = cpu_baseline(g, b, kappa, csw, fp64.get("iters", iters[0]), args.cpu_iters)
+            log("cpu baseline:", cpu)
+        except Exception as exc:  # reported, never silently replaced
+            log("cpu baseline failed:", exc)
+
+    cg_bytes = 10560 * vh * (fp64.get("iters") or iters[0])
+    fp64_ms = fp64.get("ms", ms)
+    line = {
+        "metric": "CG time to |r|/|b|<=1e-10: mixed+batched vs FP64 GPU; D-slash HBM GB/s",
+        "value": ms / 1e3,
+        "unit": "s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms,
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": {"f64": "f64", "mixed32": "f32 inner / f64 reliable", "mixed16": "f16 storage + f32 arithmetic inner / f64 reliable"}[args.solver],
+        "data": "synthetic (seeded Haar SU(3) + 5 APE sweeps, alpha %.2f; Gaussian source)" % args.alpha,
+        "config": {
+            "workload": "CG on M~^+M~ to |r|/|b|<=1e-10, clover-Wilson even-odd, %s" % "x".join(map(str, L)),
+            "lattice": list(L), "kappa": kappa, "csw": csw, "plaquette": round(plaq, 6),
+            "solver": args.solver, "iterations": iters[0], "reliable_updates": nrels[0],
+            "true_residual_rel": true_rel, "solution_rel_error_vs_fp64": sol_err,
+            "delta": args.delta, "max_inner": args.max_inner,
+            "parallelism": "single GPU",
+        },
+        "fp64_cg_s": fp64_ms / 1e3,
+        "fp64_cg_iterations": fp64.get("iters", iters[0]),
+        "speedup_vs_fp64": fp64_ms / ms,
+        "fp64_cg_algorithmic_GBs": cg_bytes / (fp64_ms * 1e-3) / 1e9,
+        "dslash_fp64_ms": ds_ms,
+        "dslash_fp64_GBs": ds_gbs,
+        "roofline": {
+            "bound": "hbm",
+            "achieved": ds_gbs,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": ds_gbs / HBM_PEAK_GBS,
+            "traffic": None,
+            "kernel": "k_dslash<SF64, false, EPI_PLAIN> (FP64 hopping matrix, one parity)",
+            "bytes_per_launch": ds_bytes,
+        },
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+    g.close()
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -25,7 +25,7 @@ LIB_PATH = os.path.join(HERE, "libbqcd_hip.so")
 
 OP = {"mtil": 0, "mtil_dag": 1, "mtdagmt": 2, "w_mult": 3, "w_mult_dag": 4, "w_dagger_w": 5}
 CLOVER = {"a": 0, "ainv": 1, "b": 2}
-PREC = {"f64": 64, "f32": 32, "f16": 16}
+PREC = {"f64": 64, "f32": 32, "f16": 16, "f16g": 18}
 
 _dp = ctypes.POINTER(ctypes.c_double)
 _ip = ctypes.POINTER(ctypes.c_int)
@@ -70,6 +70,16 @@ def lib():
         L.hb_spinor_to_device.argtypes = [_vp, _vp, _dp]
         L.hb_spinor_to_host.argtypes = [_vp, _dp, _vp]
         L.hb_last_timing.argtypes = [_vp, _dp, _ip]
+        L.hb_build_clover.argtypes = [_vp, ctypes.c_double]
+        L.hb_plaquette.argtypes = [_vp, _dp]
+        L.hb_gauge_random.argtypes = [_vp, ctypes.c_ulonglong, ctypes.c_double]
+        L.hb_gauge_smear.argtypes = [_vp, ctypes.c_int, ctypes.c_double]
+        L.hb_cg_mixed.argtypes = [_vp, ctypes.c_int, ctypes.c_double, ctypes.c_double, _dp, _dp,
+                                  ctypes.c_float, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                  ctypes.c_double, ctypes.c_int, _ip, _ip, _dp]
+        L.hb_cg_mixed_d.argtypes = [_vp, ctypes.c_int, ctypes.c_double, ctypes.c_double, _vp, _vp,
+                                    ctypes.c_float, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                    ctypes.c_double, ctypes.c_int, _ip, _ip, _dp]
         _lib = L
     return _lib
 
@@ -141,6 +151,24 @@ class Lattice:
         _check(lib().hb_save_clover(self._h, _p(a), _p(ai), _p(b)))
         return a, ai, b
 
+    def build_clover(self, csw_kappa):
+        """clover_init (clover/clover_init.F90:28) on the device from the loaded links"""
+        _check(lib().hb_build_clover(self._h, csw_kappa))
+
+    def plaquette(self):
+        """<Re Tr U_p>/3 averaged over all plaquettes"""
+        s = ctypes.c_double(0.0)
+        _check(lib().hb_plaquette(self._h, ctypes.byref(s)))
+        return s.value / (3.0 * 6.0 * 2 * self.volh)
+
+    def gauge_random(self, seed, scale=0.0):
+        """seeded synthetic links: exp(i scale w.lambda) for scale > 0, Haar SU(3) otherwise"""
+        _check(lib().hb_gauge_random(self._h, seed, scale))
+
+    def gauge_smear(self, sweeps, alpha):
+        """APE-like sweeps: U <- P_SU3[(1-alpha) U + alpha/6 staples]"""
+        _check(lib().hb_gauge_smear(self._h, sweeps, alpha))
+
     # ---- operators (host arrays) --------------------------------------------
     def d(self, e, inp, dagger=False):
         out = self.spinor()
@@ -201,6 +229,17 @@ class Lattice:
                            maxiter, log, ctypes.byref(it), ctypes.byref(tr)))
         return x, it.value, tr.value
 
+    def cg_mixed(self, op, b, kappa, rho=0.0, rest=1e-20, maxiter=10000, log=0, x0=None,
+                 precision="f16", delta=0.1, max_inner=1000):
+        """Mixed-precision CG with reliable updates; returns (x, iters, n_reliable, true |r|^2)."""
+        x = self.spinor() if x0 is None else np.array(x0, dtype=np.float64, copy=True)
+        it, nrel, tr = ctypes.c_int(0), ctypes.c_int(0), ctypes.c_double(0.0)
+        _check(lib().hb_cg_mixed(self._h, OP[op], kappa, rho, _p(x),
+                                 _p(np.ascontiguousarray(b, dtype=np.float64)), ctypes.c_float(rest),
+                                 maxiter, log, PREC[precision], delta, max_inner, ctypes.byref(it),
+                                 ctypes.byref(nrel), ctypes.byref(tr)))
+        return x, it.value, nrel.value, tr.value
+
     # ---- device-resident API (torch tensors, SoA layout) ---------------------
     def set_stream(self, stream=None):
         _check(lib().hb_set_stream(self._h, ctypes.c_void_p(stream.cuda_stream) if stream else None))
@@ -227,6 +266,14 @@ class Lattice:
         _check(lib().hb_cg_d(self._h, OP[op], kappa, rho, _tp(x), _tp(b), ctypes.c_float(rest),
                              maxiter, log, ctypes.byref(it), ctypes.byref(tr)))
         return it.value, tr.value
+
+    def cg_mixed_d(self, op, x, b, kappa, rho=0.0, rest=1e-20, maxiter=10000, log=0,
+                   precision="f16", delta=0.1, max_inner=1000):
+        it, nrel, tr = ctypes.c_int(0), ctypes.c_int(0), ctypes.c_double(0.0)
+        _check(lib().hb_cg_mixed_d(self._h, OP[op], kappa, rho, _tp(x), _tp(b), ctypes.c_float(rest),
+                                   maxiter, log, PREC[precision], delta, max_inner, ctypes.byref(it),
+                                   ctypes.byref(nrel), ctypes.byref(tr)))
+        return it.value, nrel.value, tr.value
 
     def last_timing(self):
         ms = ctypes.c_double(0.0)

@@ -19,7 +19,8 @@ LIB = os.path.join(HERE, "libbqcd_hip.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
 
-SOURCES = ["hb_context.cpp", "hb_fields.hip", "hb_dslash.hip", "hb_blas.hip", "hb_cg.hip"]
+SOURCES = ["hb_context.cpp", "hb_fields.hip", "hb_dslash.hip", "hb_blas.hip", "hb_cg.hip",
+           "hb_gauge.hip", "hb_mixed.hip"]
 
 FLAGS = [
     "-O3", "-std=c++17", "-fPIC", "--offload-arch=" + ARCH,

@@ -16,6 +16,41 @@ struct CgState {
   double res[4];   // scratch results (true residual etc.)
 };
 
+// finalize functors of the CG reductions (cg.F90:84-115)
+struct FinCgStart {  // rtr_old = |r|^2
+  CgState *st;
+  __device__ void operator()(const double (&t)[1]) const {
+    st->rtr_old = t[0];
+    st->rtr = t[0];
+    st->iter = 0;
+    st->done = 0;
+  }
+};
+struct FinCgUpdate {  // rtr = |r|^2; stop test (cg.F90:111); beta = rtr / rtr_old
+  CgState *st;
+  __device__ void operator()(const double (&t)[1]) const {
+    const double rtr = t[0];
+    st->iter += 1;
+    st->rtr = rtr;
+    if (rtr <= st->rest) {
+      st->done = 1;
+    } else if (st->iter >= st->maxiter) {
+      st->done = 2;
+    } else {
+      st->beta = rtr / st->rtr_old;
+      st->rtr_old = rtr;
+    }
+  }
+};
+struct FinStore1 {
+  double *out;
+  __device__ void operator()(const double (&t)[1]) const { out[0] = t[0]; }
+};
+struct FinStore2 {
+  double *out;
+  __device__ void operator()(const double (&t)[2]) const { out[0] = t[0]; out[1] = t[1]; }
+};
+
 int reduce_grid(const Context *c, int n);
 int clover_mult_f64(Context *c, int which, int eo, double2 *out, const double2 *in);
 int cg_start(Context *c, double2 *r, double2 *p, const double2 *b, CgState *st);

@@ -1,4 +1,4 @@
-// hb_blas.hip - spinor BLAS-1 with single-pass grid reductions, standalone
+// hb_blas.hip - spinor BLAS-1 with deterministic grid reductions, standalone
 // clover multiplies, and precision conversions of spinors, links and clover.
 #include "hb_blas.h"
 
@@ -33,102 +33,106 @@ int clover_mult_f64(Context *c, int which, int eo, double2 *out, const double2 *
 // ---------------------------------------------------------------------------
 // FP64 CG vector kernels (cg.F90:81-116); scalars live in CgState on device
 // ---------------------------------------------------------------------------
+// Vector kernels: one thread per site, the 12 components of a site in 12
+// coalesced planes (component k of site i at k*vh + i).
+#define HB_SITE_LOOP for (int k = 0; k < 12; k++)
+#define HB_AT(ptr, k) (ptr)[(size_t)(k) * vh + i]
+
 // r = b - r, p = r, rtr_old = |r|^2  (cg.F90:84-89)
 __global__ __launch_bounds__(kBlock) void k_cg_start(double2 *__restrict__ r, double2 *__restrict__ p,
-                                                     const double2 *__restrict__ b, int n,
-                                                     CgState *st, double *partials, unsigned *cnt) {
+                                                     const double2 *__restrict__ b, int vh,
+                                                     double *partials) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   double s = 0.0;
-  for (int k = i; k < n; k += gridDim.x * blockDim.x) {
-    double2 bb = b[k], rr = r[k];
-    double2 v = make_double2(bb.x - rr.x, bb.y - rr.y);
-    r[k] = v;
-    p[k] = v;
-    s += v.x * v.x + v.y * v.y;
+  if (i < vh) {
+#pragma unroll
+    HB_SITE_LOOP {
+      double2 bb = HB_AT(b, k), rr = HB_AT(r, k);
+      double2 v = make_double2(bb.x - rr.x, bb.y - rr.y);
+      HB_AT(r, k) = v;
+      HB_AT(p, k) = v;
+      s += v.x * v.x + v.y * v.y;
+    }
   }
   double vals[1] = {s};
-  grid_reduce<1>(vals, partials, cnt, [st](const double *t) {
-    st->rtr_old = t[0];
-    st->rtr = t[0];
-    st->iter = 0;
-    st->done = 0;
-  });
+  block_partials<1>(vals, partials);
 }
 
-// alpha = rtr_old / pAp; x += alpha p; r -= alpha q; rtr = |r|^2; convergence;
-// beta = rtr / rtr_old (cg.F90:96-115)
+// alpha = rtr_old / pAp; x += alpha p; r -= alpha q; rtr = |r|^2 (cg.F90:96-109)
 __global__ __launch_bounds__(kBlock) void k_cg_update(double2 *__restrict__ x, double2 *__restrict__ r,
                                                       const double2 *__restrict__ p,
-                                                      const double2 *__restrict__ q, int n,
-                                                      CgState *st, double *partials, unsigned *cnt) {
+                                                      const double2 *__restrict__ q, int vh,
+                                                      const CgState *st, double *partials) {
   if (st->done) return;
   const double alpha = st->rtr_old / st->pap;
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   double s = 0.0;
-  for (int k = i; k < n; k += gridDim.x * blockDim.x) {
-    double2 pp = p[k], qq = q[k], xx = x[k], rr = r[k];
-    xx.x += alpha * pp.x; xx.y += alpha * pp.y;
-    rr.x -= alpha * qq.x; rr.y -= alpha * qq.y;
-    x[k] = xx;
-    r[k] = rr;
-    s += rr.x * rr.x + rr.y * rr.y;
+  if (i < vh) {
+#pragma unroll
+    HB_SITE_LOOP {
+      double2 pp = HB_AT(p, k), qq = HB_AT(q, k), xx = HB_AT(x, k), rr = HB_AT(r, k);
+      xx.x += alpha * pp.x; xx.y += alpha * pp.y;
+      rr.x -= alpha * qq.x; rr.y -= alpha * qq.y;
+      HB_AT(x, k) = xx;
+      HB_AT(r, k) = rr;
+      s += rr.x * rr.x + rr.y * rr.y;
+    }
   }
   double vals[1] = {s};
-  grid_reduce<1>(vals, partials, cnt, [st](const double *t) {
-    double rtr = t[0];
-    st->iter += 1;
-    st->rtr = rtr;
-    if (rtr <= st->rest) {
-      st->done = 1;
-    } else if (st->iter >= st->maxiter) {
-      st->done = 2;
-    } else {
-      st->beta = rtr / st->rtr_old;
-      st->rtr_old = rtr;
-    }
-  });
+  block_partials<1>(vals, partials);
 }
 
 // p = r + beta p  (sc_xpby, sc.F90:184-201)
 __global__ __launch_bounds__(kBlock) void k_cg_pupdate(double2 *__restrict__ p, const double2 *__restrict__ r,
-                                                       int n, const CgState *st) {
+                                                       int vh, const CgState *st) {
   if (st->done) return;
   const double beta = st->beta;
   int i = blockIdx.x * blockDim.x + threadIdx.x;
-  for (int k = i; k < n; k += gridDim.x * blockDim.x) {
-    double2 pp = p[k], rr = r[k];
-    p[k] = make_double2(rr.x + beta * pp.x, rr.y + beta * pp.y);
+  if (i >= vh) return;
+#pragma unroll
+  HB_SITE_LOOP {
+    double2 pp = HB_AT(p, k), rr = HB_AT(r, k);
+    HB_AT(p, k) = make_double2(rr.x + beta * pp.x, rr.y + beta * pp.y);
   }
 }
 
-// out = b - out, result |out|^2 -> *res
+// out = b - out, result |out|^2
 __global__ __launch_bounds__(kBlock) void k_residual(double2 *__restrict__ out, const double2 *__restrict__ b,
-                                                     int n, double *res, double *partials, unsigned *cnt) {
+                                                     int vh, double *partials) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   double s = 0.0;
-  for (int k = i; k < n; k += gridDim.x * blockDim.x) {
-    double2 bb = b[k], o = out[k];
-    double2 v = make_double2(bb.x - o.x, bb.y - o.y);
-    out[k] = v;
-    s += v.x * v.x + v.y * v.y;
+  if (i < vh) {
+#pragma unroll
+    HB_SITE_LOOP {
+      double2 bb = HB_AT(b, k), o = HB_AT(out, k);
+      double2 v = make_double2(bb.x - o.x, bb.y - o.y);
+      HB_AT(out, k) = v;
+      s += v.x * v.x + v.y * v.y;
+    }
   }
   double vals[1] = {s};
-  grid_reduce<1>(vals, partials, cnt, [res](const double *t) { res[0] = t[0]; });
+  block_partials<1>(vals, partials);
 }
 
 // Re<x, y> and |x|^2 of FP64 fields
 __global__ __launch_bounds__(kBlock) void k_dot_norm(const double2 *__restrict__ x, const double2 *__restrict__ y,
-                                                     int n, double *res, double *partials, unsigned *cnt) {
+                                                     int vh, double *partials) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   double s0 = 0.0, s1 = 0.0;
-  for (int k = i; k < n; k += gridDim.x * blockDim.x) {
-    double2 a = x[k], b = y[k];
-    s0 += a.x * b.x + a.y * b.y;
-    s1 += a.x * a.x + a.y * a.y;
+  if (i < vh) {
+#pragma unroll
+    HB_SITE_LOOP {
+      double2 a = HB_AT(x, k), b = HB_AT(y, k);
+      s0 += a.x * b.x + a.y * b.y;
+      s1 += a.x * a.x + a.y * a.y;
+    }
   }
   double vals[2] = {s0, s1};
-  grid_reduce<2>(vals, partials, cnt, [res](const double *t) { res[0] = t[0]; res[1] = t[1]; });
+  block_partials<2>(vals, partials);
 }
+
+#undef HB_SITE_LOOP
+#undef HB_AT
 
 __global__ void k_zero(double2 *p, size_t n) {
   size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -147,36 +151,40 @@ int reduce_grid(const Context *c, int n) {
 }
 
 int cg_start(Context *c, double2 *r, double2 *p, const double2 *b, CgState *st) {
-  int n = 12 * c->g.vh;
-  k_cg_start<<<reduce_grid(c, n), kBlock, 0, c->stream>>>(r, p, b, n, st, c->red, c->red_count);
+  const int vh = c->g.vh, grid = nblocks(vh);
+  k_cg_start<<<grid, kBlock, 0, c->stream>>>(r, p, b, vh, c->red);
+  k_finalize<1><<<1, 256, 0, c->stream>>>(c->red, grid, nullptr, FinCgStart{st});
   HB_HIP(hipGetLastError());
   return HB_OK;
 }
 
 int cg_update(Context *c, double2 *x, double2 *r, const double2 *p, const double2 *q, CgState *st) {
-  int n = 12 * c->g.vh;
-  k_cg_update<<<reduce_grid(c, n), kBlock, 0, c->stream>>>(x, r, p, q, n, st, c->red, c->red_count + 1);
+  const int vh = c->g.vh, grid = nblocks(vh);
+  k_cg_update<<<grid, kBlock, 0, c->stream>>>(x, r, p, q, vh, st, c->red);
+  k_finalize<1><<<1, 256, 0, c->stream>>>(c->red, grid, &st->done, FinCgUpdate{st});
   HB_HIP(hipGetLastError());
   return HB_OK;
 }
 
 int cg_pupdate(Context *c, double2 *p, const double2 *r, const CgState *st) {
-  int n = 12 * c->g.vh;
-  k_cg_pupdate<<<nblocks(n), kBlock, 0, c->stream>>>(p, r, n, st);
+  const int vh = c->g.vh;
+  k_cg_pupdate<<<nblocks(vh), kBlock, 0, c->stream>>>(p, r, vh, st);
   HB_HIP(hipGetLastError());
   return HB_OK;
 }
 
 int residual(Context *c, double2 *out, const double2 *b, double *res_d) {
-  int n = 12 * c->g.vh;
-  k_residual<<<reduce_grid(c, n), kBlock, 0, c->stream>>>(out, b, n, res_d, c->red, c->red_count + 2);
+  const int vh = c->g.vh, grid = nblocks(vh);
+  k_residual<<<grid, kBlock, 0, c->stream>>>(out, b, vh, c->red);
+  k_finalize<1><<<1, 256, 0, c->stream>>>(c->red, grid, nullptr, FinStore1{res_d});
   HB_HIP(hipGetLastError());
   return HB_OK;
 }
 
 int dot_norm(Context *c, const double2 *x, const double2 *y, double *res_d) {
-  int n = 12 * c->g.vh;
-  k_dot_norm<<<reduce_grid(c, n), kBlock, 0, c->stream>>>(x, y, n, res_d, c->red, c->red_count + 3);
+  const int vh = c->g.vh, grid = nblocks(vh);
+  k_dot_norm<<<grid, kBlock, 0, c->stream>>>(x, y, vh, c->red);
+  k_finalize<2><<<1, 256, 0, c->stream>>>(c->red, grid, nullptr, FinStore2{res_d});
   HB_HIP(hipGetLastError());
   return HB_OK;
 }

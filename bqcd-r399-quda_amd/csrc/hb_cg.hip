@@ -19,7 +19,6 @@ DslashArgs<SF64> dslash_args64(Context *c, int out_parity, double2 *out, const d
   a.in = {const_cast<double2 *>(in), nullptr};
   a.U = c->gauge;
   a.partials = c->red;
-  a.counter = c->red_count + 4;
   return a;
 }
 

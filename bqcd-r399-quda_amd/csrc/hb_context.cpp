@@ -99,11 +99,9 @@ int hb_init(const int L[4], const int bc[4], int device, hb_context **out) {
     return HB_E_HIP;
   }
   c->stream = c->own_stream;
-  // reduction buffers: up to 8 values per block for the largest grids used
-  c->red_blocks = nblocks(c->g.vh) + 64;
+  // reduction partials: up to 16 values per block of a site-parallel grid
+  c->red_blocks = nblocks(2 * c->g.vh) + 64;
   e = hipMalloc(&c->red, sizeof(double) * 16 * (size_t)c->red_blocks);
-  if (e == hipSuccess) e = hipMalloc(&c->red_count, sizeof(unsigned int) * 64);
-  if (e == hipSuccess) e = hipMemset(c->red_count, 0, sizeof(unsigned int) * 64);
   if (e != hipSuccess) {
     set_error("hb_init: %s", hipGetErrorString(e));
     hb_finalize(reinterpret_cast<hb_context *>(c));
@@ -128,7 +126,6 @@ int hb_finalize(hb_context *ctx) {
   for (auto *p : c->clov16n) (void)hipFree(p);
   for (auto &f : c->scratch) (void)hipFree(f.p);
   (void)hipFree(c->red);
-  (void)hipFree(c->red_count);
   (void)hipFree(c->dev_stage);
   if (c->host_stage) (void)hipHostFree(c->host_stage);
   if (c->own_stream) (void)hipStreamDestroy(c->own_stream);

@@ -19,8 +19,8 @@ namespace hb {
 
 // One hop: acc += (1 + s gamma_mu) U psi(x +- mu), U = U_mu(x) forward or
 // U_mu(x - mu)^+ backward, with the boundary sign when the hop wraps.
-template <class S, int MU, bool FWD, int SG>
-__device__ __forceinline__ void hop(typename S::V (&acc)[12], const DslashArgs<S> &a,
+template <class S, class G, int MU, bool FWD, int SG>
+__device__ __forceinline__ void hop(typename S::V (&acc)[12], const DslashArgs<S, G> &a,
                                     const SpinorView<S> &in, const Geom &g, int e, int i,
                                     const int (&xc)[4]) {
   using V = typename S::V;
@@ -57,10 +57,10 @@ __device__ __forceinline__ void hop(typename S::V (&acc)[12], const DslashArgs<S
   }
   V u[9];
   if (FWD) {
-    load_link<S>(u, a.U, MU, e, i, vh);
+    load_link<G>(u, a.U, MU, e, i, vh);
     su3_mul_half(w, u, h);
   } else {
-    load_link<S>(u, a.U, MU, 1 - e, j, vh);
+    load_link<G>(u, a.U, MU, 1 - e, j, vh);
     su3_adj_mul_half(w, u, h);
   }
   reconstruct<MU, SG>(acc, w);
@@ -75,14 +75,14 @@ template <class V> __device__ __forceinline__ void pin(V (&acc)[12]) {
 }
 #define HB_SCHED_FENCE() do { pin(acc); __builtin_amdgcn_sched_barrier(0); } while (0)
 
-template <class S, bool DAG, int EPI>
-__global__ __launch_bounds__(kBlock) void k_dslash(DslashArgs<S> a) {
+template <class S, class G, bool DAG, int EPI>
+__global__ __launch_bounds__(kBlock) void k_dslash(DslashArgs<S, G> a) {
   using V = typename S::V;
   using A = typename S::A;
   const Geom g = a.g;
   const int vh = g.vh;
   if (a.skip && *a.skip) return;
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int i = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
   V acc[12];
 #pragma unroll
   for (int k = 0; k < 12; k++) acc[k] = cmk<V>((A)0, (A)0);
@@ -95,14 +95,14 @@ __global__ __launch_bounds__(kBlock) void k_dslash(DslashArgs<S> a) {
     // forward projector sign: (1 - gamma) for D, (1 + gamma) for D^+
     constexpr int SF = DAG ? 1 : -1;
     constexpr int SB = -SF;
-    hop<S, 3, true, SF>(acc, a, in, g, e, i, xc);  HB_SCHED_FENCE();
-    hop<S, 3, false, SB>(acc, a, in, g, e, i, xc); HB_SCHED_FENCE();
-    hop<S, 2, true, SF>(acc, a, in, g, e, i, xc);  HB_SCHED_FENCE();
-    hop<S, 2, false, SB>(acc, a, in, g, e, i, xc); HB_SCHED_FENCE();
-    hop<S, 1, true, SF>(acc, a, in, g, e, i, xc);  HB_SCHED_FENCE();
-    hop<S, 1, false, SB>(acc, a, in, g, e, i, xc); HB_SCHED_FENCE();
-    hop<S, 0, true, SF>(acc, a, in, g, e, i, xc);  HB_SCHED_FENCE();
-    hop<S, 0, false, SB>(acc, a, in, g, e, i, xc); HB_SCHED_FENCE();
+    hop<S, G, 3, true, SF>(acc, a, in, g, e, i, xc);  HB_SCHED_FENCE();
+    hop<S, G, 3, false, SB>(acc, a, in, g, e, i, xc); HB_SCHED_FENCE();
+    hop<S, G, 2, true, SF>(acc, a, in, g, e, i, xc);  HB_SCHED_FENCE();
+    hop<S, G, 2, false, SB>(acc, a, in, g, e, i, xc); HB_SCHED_FENCE();
+    hop<S, G, 1, true, SF>(acc, a, in, g, e, i, xc);  HB_SCHED_FENCE();
+    hop<S, G, 1, false, SB>(acc, a, in, g, e, i, xc); HB_SCHED_FENCE();
+    hop<S, G, 0, true, SF>(acc, a, in, g, e, i, xc);  HB_SCHED_FENCE();
+    hop<S, G, 0, false, SB>(acc, a, in, g, e, i, xc); HB_SCHED_FENCE();
   }
 
   if constexpr (EPI == EPI_PLAIN) {
@@ -111,7 +111,7 @@ __global__ __launch_bounds__(kBlock) void k_dslash(DslashArgs<S> a) {
     if (active) {
       if (a.C) {
         V r[12];
-        clover_apply<S, 0>(r, acc, a.C, a.Cn, e, i, vh);
+        clover_apply<G, 0>(r, acc, a.C, a.Cn, e, i, vh);
         store_spinor<S>(a.out, i, vh, r);
       } else {
         store_spinor<S>(a.out, i, vh, acc);
@@ -125,7 +125,7 @@ __global__ __launch_bounds__(kBlock) void k_dslash(DslashArgs<S> a) {
       // the hop sum is complete; keep it from being interleaved with the
       // clover loads
       HB_SCHED_FENCE();
-      if (a.C) clover_apply<S, 0>(r, x0, a.C, a.Cn, e, i, vh);
+      if (a.C) clover_apply<G, 0>(r, x0, a.C, a.Cn, e, i, vh);
       else {
 #pragma unroll
         for (int k = 0; k < 12; k++) r[k] = x0[k];
@@ -146,16 +146,15 @@ __global__ __launch_bounds__(kBlock) void k_dslash(DslashArgs<S> a) {
     }
     if constexpr (EPI == EPI_MTDOT) {
       double vals[1] = {dot};
-      double *res = a.dot_out;
-      grid_reduce<1>(vals, a.partials, a.counter, [res](const double *s) { res[0] = s[0]; });
+      block_partials<1>(vals, a.partials);
     }
   }
 }
 
-template <class S>
-int launch_dslash(const DslashArgs<S> &a, bool dag, int epi, hipStream_t st) {
+template <class S, class G>
+int launch_dslash(const DslashArgs<S, G> &a, bool dag, int epi, hipStream_t st) {
   dim3 grid(nblocks(a.g.vh)), block(kBlock);
-#define HB_DS(D, E) hipLaunchKernelGGL((k_dslash<S, D, E>), grid, block, 0, st, a)
+#define HB_DS(D, E) hipLaunchKernelGGL((k_dslash<S, G, D, E>), grid, block, 0, st, a)
   if (!dag) {
     switch (epi) {
       case EPI_PLAIN: HB_DS(false, EPI_PLAIN); break;
@@ -172,12 +171,16 @@ int launch_dslash(const DslashArgs<S> &a, bool dag, int epi, hipStream_t st) {
     }
   }
 #undef HB_DS
+  if (epi == EPI_MTDOT)
+    k_finalize<1><<<1, 256, 0, st>>>(a.partials, (int)grid.x, a.skip, FinStore1{a.dot_out});
   HB_HIP(hipGetLastError());
   return HB_OK;
 }
 
-template int launch_dslash<SF64>(const DslashArgs<SF64> &, bool, int, hipStream_t);
-template int launch_dslash<SF32>(const DslashArgs<SF32> &, bool, int, hipStream_t);
-template int launch_dslash<SF16>(const DslashArgs<SF16> &, bool, int, hipStream_t);
+template int launch_dslash<SF64, SF64>(const DslashArgs<SF64, SF64> &, bool, int, hipStream_t);
+template int launch_dslash<SF32, SF32>(const DslashArgs<SF32, SF32> &, bool, int, hipStream_t);
+template int launch_dslash<SF16, SF16>(const DslashArgs<SF16, SF16> &, bool, int, hipStream_t);
+// FP32 spinors with FP16 links and clover
+template int launch_dslash<SF32, SF16>(const DslashArgs<SF32, SF16> &, bool, int, hipStream_t);
 
 }  // namespace hb

@@ -102,15 +102,14 @@ struct Context {
   // low-precision copies (built lazily from the FP64 fields)
   float2 *gauge32 = nullptr;
   float2 *clov32[2] = {nullptr, nullptr};          // A, A^-1
-  __half2 *gauge16 = nullptr;
-  __half2 *clov16[2] = {nullptr, nullptr};
+  short2 *gauge16 = nullptr;                       // 16-bit fixed point
+  short2 *clov16[2] = {nullptr, nullptr};
   float *clov16n[2] = {nullptr, nullptr};          // per-site per-block scale
   uint64_t gauge_version = 1, clover_version = 1;
   uint64_t lp_gauge_version = 0, lp_clover_version = 0;
   // scratch
   std::vector<Field> scratch;
-  double *red = nullptr;             // reduction partials
-  unsigned int *red_count = nullptr; // last-block counters
+  double *red = nullptr;             // per-block partial sums of the reductions
   int red_blocks = 0;
   void *host_stage = nullptr;        // pinned staging
   size_t host_stage_bytes = 0;
@@ -125,6 +124,16 @@ int ensure_host_stage(Context *c, size_t bytes);
 void *scratch(Context *c, int slot, size_t bytes, int *rc);
 
 constexpr int kBlock = 256;
+
+// The hardware hands workgroup b to XCD (b mod 8).  This bijection of
+// [0, nblk) gives each of the 8 XCDs (each with its own L2) one contiguous
+// range of sites instead of every 8th block, so that neighbour reuse between
+// adjacent blocks stays inside one L2.
+__device__ __forceinline__ int xcd_block(int b, int nblk) {
+  const int q = nblk >> 3, r = nblk & 7;
+  const int x = b & 7, k = b >> 3;
+  return x * q + (x < r ? x : r) + k;
+}
 inline int nblocks(int n, int b = kBlock) { return (n + b - 1) / b; }
 
 // ----------------------------------------------------------------------------
@@ -186,14 +195,14 @@ __device__ __forceinline__ double block_sum(double v, double *lds) {
   return s;
 }
 
-// Single-pass deterministic grid reduction of NV values: every block writes its
-// partials, the last block to finish sums them in block order and calls fin(sums)
-// from thread 0.  counter must start at 0; the last block resets it.
-template <int NV, typename Fin>
-__device__ __forceinline__ void grid_reduce(double (&v)[NV], double *partials,
-                                            unsigned int *counter, Fin fin) {
+// Grid reductions in two launches, no fences: every block of the producing
+// kernel stores its NV partial sums (plain stores, visible to later launches
+// on the stream), then a one-block finalize kernel sums them in a fixed order
+// -- deterministic, and no agent-scope release fence (an L2 write-back on
+// CDNA) per block.
+template <int NV>
+__device__ __forceinline__ void block_partials(const double (&v)[NV], double *partials) {
   __shared__ double lds[NV][16];
-  __shared__ bool last;
 #pragma unroll
   for (int k = 0; k < NV; k++) {
     double s = wave_sum(v[k]);
@@ -208,41 +217,40 @@ __device__ __forceinline__ void grid_reduce(double (&v)[NV], double *partials,
       for (int w = 0; w < nw; w++) s += lds[k][w];
       partials[(size_t)k * gridDim.x + blockIdx.x] = s;
     }
-    __threadfence();
-    unsigned int prev = atomicAdd(counter, 1u);
-    last = (prev == gridDim.x - 1);
   }
-  __syncthreads();
-  if (!last) return;
-  __threadfence();
-  // the last block sums all partials (in fixed order per thread, then tree)
+}
+
+// one block: sums nparts partials per value (fixed order) and calls f(sums)
+// from thread 0; `skip` gates it like the producing kernel
+template <int NV, class F>
+__global__ __launch_bounds__(256) void k_finalize(const double *__restrict__ partials, int nparts,
+                                                  const int *skip, F f) {
+  if (skip && *skip) return;
+  __shared__ double lds[NV][4];
   double acc[NV];
 #pragma unroll
   for (int k = 0; k < NV; k++) {
     double s = 0.0;
-    for (int b = threadIdx.x; b < (int)gridDim.x; b += blockDim.x)
-      s += partials[(size_t)k * gridDim.x + b];
-    acc[k] = s;
-  }
-  __syncthreads();
-#pragma unroll
-  for (int k = 0; k < NV; k++) {
-    double s = wave_sum(acc[k]);
+    for (int b = threadIdx.x; b < nparts; b += blockDim.x) s += partials[(size_t)k * nparts + b];
+    s = wave_sum(s);
     if ((threadIdx.x & 63) == 0) lds[k][threadIdx.x >> 6] = s;
   }
   __syncthreads();
   if (threadIdx.x == 0) {
-    int nw = (blockDim.x + 63) >> 6;
-    double tot[NV];
 #pragma unroll
-    for (int k = 0; k < NV; k++) {
-      double s = 0.0;
-      for (int w = 0; w < nw; w++) s += lds[k][w];
-      tot[k] = s;
-    }
-    *counter = 0u;
-    fin(tot);
+    for (int k = 0; k < NV; k++) acc[k] = lds[k][0] + lds[k][1] + lds[k][2] + lds[k][3];
+    f(acc);
   }
 }
+
+// finalize functor: store the sums
+struct FinStore {
+  double *out;
+  template <int NV> __device__ void operator()(const double (&s)[NV]) const {
+#pragma unroll
+    for (int k = 0; k < NV; k++) out[k] = s[k];
+  }
+  __device__ void operator()(const double *s) const { out[0] = s[0]; }
+};
 
 }  // namespace hb

@@ -20,14 +20,17 @@ typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
 __device__ __forceinline__ rsrc_t make_rsrc(const void *p, uint32_t bytes) {
   return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(p), (short)0, (int)bytes, 0x00020000);
 }
+// cache policy of buffer loads: 0 = default, kNT = non-temporal (streamed
+// data read once per kernel, e.g. links, so that it does not evict the
+// neighbour spinors that other sites reuse)
+constexpr int kNT = 2;
+template <int POL = 0>
 __device__ __forceinline__ double2 bld_f64x2(rsrc_t r, uint32_t vo, uint32_t so) {
-  return __builtin_bit_cast(double2, __builtin_amdgcn_raw_buffer_load_b128(r, vo, so, 0));
+  return __builtin_bit_cast(double2, __builtin_amdgcn_raw_buffer_load_b128(r, vo, so, POL));
 }
+template <int POL = 0>
 __device__ __forceinline__ float2 bld_f32x2(rsrc_t r, uint32_t vo, uint32_t so) {
-  return __builtin_bit_cast(float2, __builtin_amdgcn_raw_buffer_load_b64(r, vo, so, 0));
-}
-__device__ __forceinline__ __half2 bld_f16x2(rsrc_t r, uint32_t vo, uint32_t so) {
-  return __builtin_bit_cast(__half2, __builtin_amdgcn_raw_buffer_load_b32(r, vo, so, 0));
+  return __builtin_bit_cast(float2, __builtin_amdgcn_raw_buffer_load_b64(r, vo, so, POL));
 }
 __device__ __forceinline__ float bld_f32(rsrc_t r, uint32_t vo, uint32_t so) {
   return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, vo, so, 0));
@@ -44,7 +47,8 @@ struct SF64 {
   static constexpr int kBytes = 16;
   __device__ __forceinline__ static V ld(const ST *p, size_t k) { return p[k]; }
   __device__ __forceinline__ static void st(ST *p, size_t k, V v) { p[k] = v; }
-  __device__ __forceinline__ static V bld(rsrc_t r, uint32_t vo, uint32_t so) { return bld_f64x2(r, vo, so); }
+  template <int POL = 0>
+  __device__ __forceinline__ static V bld(rsrc_t r, uint32_t vo, uint32_t so) { return bld_f64x2<POL>(r, vo, so); }
 };
 
 struct SF32 {
@@ -55,21 +59,34 @@ struct SF32 {
   static constexpr int kBytes = 8;
   __device__ __forceinline__ static V ld(const ST *p, size_t k) { return p[k]; }
   __device__ __forceinline__ static void st(ST *p, size_t k, V v) { p[k] = v; }
-  __device__ __forceinline__ static V bld(rsrc_t r, uint32_t vo, uint32_t so) { return bld_f32x2(r, vo, so); }
+  template <int POL = 0>
+  __device__ __forceinline__ static V bld(rsrc_t r, uint32_t vo, uint32_t so) { return bld_f32x2<POL>(r, vo, so); }
 };
 
-// FP16 storage, FP32 arithmetic; spinors and clover blocks carry a per-site
-// FP32 scale (the maximum magnitude of the site's components)
+// 16-bit storage, FP32 arithmetic: signed 16-bit fixed point over [-1, 1]
+// (resolution 2^-15, uniformly -- IEEE half would give only 2^-11 relative and
+// ~1e-3 absolute spacing near 1, too coarse for links of the operator).
+// Links are stored as is (|U_ij| <= 1); spinors and clover blocks carry a
+// per-site FP32 scale (the site's largest magnitude).
 struct SF16 {
   using A = float;
   using V = float2;
-  using ST = __half2;
+  using ST = short2;
   static constexpr bool kNorm = true;
   static constexpr int kBytes = 4;
-  __device__ __forceinline__ static V ld(const ST *p, size_t k) { return __half22float2(p[k]); }
-  __device__ __forceinline__ static void st(ST *p, size_t k, V v) { p[k] = __float22half2_rn(v); }
+  static constexpr float kScale = 32767.0f;
+  static constexpr float kInv = 1.0f / 32767.0f;
+  __device__ __forceinline__ static V cvt(ST s) { return make_float2(s.x * kInv, s.y * kInv); }
+  __device__ __forceinline__ static short q(float v) {
+    float r = rintf(v * kScale);
+    r = fminf(fmaxf(r, -kScale), kScale);
+    return (short)r;
+  }
+  __device__ __forceinline__ static V ld(const ST *p, size_t k) { return cvt(p[k]); }
+  __device__ __forceinline__ static void st(ST *p, size_t k, V v) { p[k] = make_short2(q(v.x), q(v.y)); }
+  template <int POL = 0>
   __device__ __forceinline__ static V bld(rsrc_t r, uint32_t vo, uint32_t so) {
-    return __half22float2(bld_f16x2(r, vo, so));
+    return cvt(__builtin_bit_cast(ST, __builtin_amdgcn_raw_buffer_load_b32(r, vo, so, POL)));
   }
 };
 
@@ -146,14 +163,14 @@ __device__ __forceinline__ void store_spinor(const SpinorPtr<S> &f, int i, int v
 // ----------------------------------------------------------------------------
 // links: U[4][2][9][vh], element 3*row + col; one buffer per (mu, eo) slab
 // ----------------------------------------------------------------------------
-template <class S>
+template <class S, int POL = kNT>
 __device__ __forceinline__ void load_link(typename S::V (&u)[9], const typename S::ST *U, int mu,
                                           int eo, int i, int vh) {
   const uint32_t plane = (uint32_t)vh * S::kBytes;
   rsrc_t r = make_rsrc(U + (size_t)((mu * 2 + eo) * 9) * vh, plane * 9u);
   const uint32_t vo = (uint32_t)i * S::kBytes;
 #pragma unroll
-  for (int k = 0; k < 9; k++) u[k] = S::bld(r, vo, plane * k);
+  for (int k = 0; k < 9; k++) u[k] = S::template bld<POL>(r, vo, plane * k);
 }
 
 // w = U h  (h: 3 colours x 2 spins, index 2*c + s)
@@ -309,7 +326,7 @@ __device__ __forceinline__ void load_clover_block(typename S::V (&w)[18], const 
   rsrc_t r = make_rsrc(C + (size_t)(eo * 36 + J * 18) * vh, plane * 18u);
   const uint32_t vo = (uint32_t)i * S::kBytes;
 #pragma unroll
-  for (int k = 0; k < 18; k++) w[k] = S::bld(r, vo, plane * k);
+  for (int k = 0; k < 18; k++) w[k] = S::template bld<kNT>(r, vo, plane * k);
   if constexpr (S::kNorm) {
     float n = Cn[(size_t)(eo * 2 + J) * vh + i];
 #pragma unroll
@@ -318,17 +335,18 @@ __device__ __forceinline__ void load_clover_block(typename S::V (&w)[18], const 
 }
 
 // out = A in (Dirac basis) with A stored as two chiral blocks of A/2
-// (clover_mult_a.F90:44-95); mode 0 = Hermitian product, 1 = LDL^+ solve
-template <class S, int MODE, class V>
+// (clover_mult_a.F90:44-95); mode 0 = Hermitian product, 1 = LDL^+ solve.
+// SG is the clover storage policy, V the arithmetic complex type.
+template <class SG, int MODE, class V>
 __device__ __forceinline__ void clover_apply(V (&out)[12], const V (&in)[12],
-                                             const typename S::ST *C, const float *Cn, int eo,
+                                             const typename SG::ST *C, const float *Cn, int eo,
                                              int i, int vh) {
   V xp[6], y1[6], y2[6];
 #pragma unroll
   for (int m = 0; m < 6; m++) xp[m] = cadd(in[m], in[m + 6]);
   {
-    typename S::V w[18];
-    load_clover_block<S>(w, C, Cn, eo, 0, i, vh);
+    typename SG::V w[18];
+    load_clover_block<SG>(w, C, Cn, eo, 0, i, vh);
     V ww[18];
 #pragma unroll
     for (int k = 0; k < 18; k++) ww[k] = cmk<V>(w[k].x, w[k].y);
@@ -341,8 +359,8 @@ __device__ __forceinline__ void clover_apply(V (&out)[12], const V (&in)[12],
 #pragma unroll
   for (int m = 0; m < 6; m++) xp[m] = csub(in[m], in[m + 6]);
   {
-    typename S::V w[18];
-    load_clover_block<S>(w, C, Cn, eo, 1, i, vh);
+    typename SG::V w[18];
+    load_clover_block<SG>(w, C, Cn, eo, 1, i, vh);
     V ww[18];
 #pragma unroll
     for (int k = 0; k < 18; k++) ww[k] = cmk<V>(w[k].x, w[k].y);

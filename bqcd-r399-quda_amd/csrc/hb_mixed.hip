@@ -1,0 +1,476 @@
+// hb_mixed.hip - mixed-precision CG with reliable updates.
+//
+// The inner iteration is the reference's CG (cg.F90:93-117) on the operator
+// built from FP32 or 16-bit copies of the links and clover term, with FP32
+// arithmetic and the search direction, residual and operator temporaries in
+// the same storage precision (16-bit spinors carry a per-site FP32 scale).  The
+// dot products accumulate in FP64.  A reliable update (defect correction)
+// folds the low-precision solution increment into the FP64 solution and
+// replaces the iterated residual by the FP64 true residual b - A x whenever the
+// iterated residual has dropped by `delta` against its maximum since the last
+// update, after `max_inner` iterations, or when it reaches the stopping bound.
+// The solve stops only on the FP64 true residual (|r|^2 <= rest).
+#include <cmath>
+
+#include "hb_ops.h"
+
+namespace hb {
+
+struct MixState {
+  double rtr, rtr_new, pap, beta, maxr;
+  double rest, delta;
+  int iter, since, max_inner, maxiter;
+  int flag;      // 0 iterate, 1 reliable update due, 2 converged, 3 maxiter reached
+  int n_reliable;
+  double res[4];
+};
+
+// ---- precision conversion ----------------------------------------------------
+template <class S>
+__global__ void k_cvt_field(typename S::ST *__restrict__ dst, const double2 *__restrict__ src, size_t n) {
+  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  for (size_t k = i; k < n; k += (size_t)gridDim.x * blockDim.x) {
+    double2 v = src[k];
+    S::st(dst, k, cmk<typename S::V>((typename S::A)v.x, (typename S::A)v.y));
+  }
+}
+
+// clover in 16-bit storage: per site and block, scale by the largest magnitude
+__global__ void k_cvt_clover16(short2 *__restrict__ dst, float *__restrict__ nrm,
+                               const double2 *__restrict__ src, int vh) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  int eb = blockIdx.y;  // eo*2 + J
+  if (i >= vh) return;
+  const double2 *s = src + (size_t)eb * 18 * vh + i;
+  double m = 0.0;
+  for (int k = 0; k < 18; k++) m = fmax(m, fmax(fabs(s[(size_t)k * vh].x), fabs(s[(size_t)k * vh].y)));
+  float fm = (float)m;
+  float inv = fm > 0.f ? 1.0f / fm : 0.f;
+  nrm[(size_t)eb * vh + i] = fm;
+  for (int k = 0; k < 18; k++) {
+    double2 v = s[(size_t)k * vh];
+    SF16::st(dst, (size_t)eb * 18 * vh + (size_t)k * vh + i, make_float2((float)v.x * inv, (float)v.y * inv));
+  }
+}
+
+// spinor FP64 -> S (thread per site: FP16 needs the site's norm)
+template <class S>
+__global__ void k_cvt_spinor(SpinorPtr<S> dst, const double2 *__restrict__ src, int vh) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= vh) return;
+  double2 v[12];
+#pragma unroll
+  for (int k = 0; k < 12; k++) v[k] = src[(size_t)k * vh + i];
+  store_spinor<S>(dst, i, vh, v);
+}
+
+// ---- inner iteration kernels -----------------------------------------------------
+// alpha = rtr / pAp; x_lo += alpha p; r -= alpha q; rtr_new = |r|^2; decide
+template <class S>
+__global__ __launch_bounds__(kBlock) void k_mix_update(float2 *__restrict__ xlo, SpinorPtr<S> r,
+                                                       SpinorPtr<S> p, SpinorPtr<S> q, int vh,
+                                                       const MixState *st, double *partials) {
+  using V = typename S::V;
+  if (st->flag) return;
+  const float alpha = (float)(st->rtr / st->pap);
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  double s = 0.0;
+  if (i < vh) {
+    V pv[12], qv[12], rv[12];
+    load_spinor<S>(pv, p, i, vh);
+    load_spinor<S>(qv, q, i, vh);
+    load_spinor<S>(rv, r, i, vh);
+#pragma unroll
+    for (int k = 0; k < 12; k++) {
+      float2 xv = xlo[(size_t)k * vh + i];
+      xv.x += alpha * pv[k].x;
+      xv.y += alpha * pv[k].y;
+      xlo[(size_t)k * vh + i] = xv;
+      rv[k].x -= alpha * qv[k].x;
+      rv[k].y -= alpha * qv[k].y;
+      s += (double)rv[k].x * rv[k].x + (double)rv[k].y * rv[k].y;
+    }
+    store_spinor<S>(r, i, vh, rv);
+  }
+  double vals[1] = {s};
+  block_partials<1>(vals, partials);
+}
+
+// iterated |r|^2 -> reliable-update decision or beta (cg.F90:109-115)
+struct FinMixUpdate {
+  MixState *st;
+  __device__ void operator()(const double (&t)[1]) const {
+    const double rn = t[0];
+    st->iter += 1;
+    st->since += 1;
+    st->rtr_new = rn;
+    const double rnorm = sqrt(rn);
+    if (rnorm > st->maxr) st->maxr = rnorm;
+    if (rn <= st->rest || rnorm < st->delta * st->maxr || st->since >= st->max_inner ||
+        st->iter >= st->maxiter) {
+      st->flag = 1;
+    } else {
+      st->beta = rn / st->rtr;
+      st->rtr = rn;
+    }
+  }
+};
+
+// p = r + beta p
+template <class S>
+__global__ __launch_bounds__(kBlock) void k_mix_pupdate(SpinorPtr<S> p, SpinorPtr<S> r, int vh,
+                                                        const MixState *st) {
+  using V = typename S::V;
+  if (st->flag) return;
+  const float beta = (float)st->beta;
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= vh) return;
+  V pv[12], rv[12];
+  load_spinor<S>(pv, p, i, vh);
+  load_spinor<S>(rv, r, i, vh);
+#pragma unroll
+  for (int k = 0; k < 12; k++) {
+    pv[k].x = rv[k].x + beta * pv[k].x;
+    pv[k].y = rv[k].y + beta * pv[k].y;
+  }
+  store_spinor<S>(p, i, vh, pv);
+}
+
+// ---- reliable update kernels ------------------------------------------------------
+// x += x_lo; x_lo = 0
+__global__ void k_mix_accum(double2 *__restrict__ x, float2 *__restrict__ xlo, size_t n) {
+  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  for (size_t k = i; k < n; k += (size_t)gridDim.x * blockDim.x) {
+    float2 d = xlo[k];
+    double2 v = x[k];
+    x[k] = make_double2(v.x + (double)d.x, v.y + (double)d.y);
+    xlo[k] = make_float2(0.f, 0.f);
+  }
+}
+
+// r = b - Ax (in place in ax), |r|^2; then r_lo = r and the scalar bookkeeping
+template <class S>
+__global__ __launch_bounds__(kBlock) void k_mix_true_residual(double2 *__restrict__ ax,
+                                                              const double2 *__restrict__ b,
+                                                              SpinorPtr<S> rlo, int vh,
+                                                              double *partials) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  double s = 0.0;
+  if (i < vh) {
+    double2 v[12];
+#pragma unroll
+    for (int k = 0; k < 12; k++) {
+      double2 bb = b[(size_t)k * vh + i], a = ax[(size_t)k * vh + i];
+      v[k] = make_double2(bb.x - a.x, bb.y - a.y);
+      ax[(size_t)k * vh + i] = v[k];
+      s += v[k].x * v[k].x + v[k].y * v[k].y;
+    }
+    store_spinor<S>(rlo, i, vh, v);
+  }
+  double vals[1] = {s};
+  block_partials<1>(vals, partials);
+}
+
+// true |r|^2 -> convergence, or restart of the inner iteration
+struct FinMixTrue {
+  MixState *st;
+  int first;
+  __device__ void operator()(const double (&t)[1]) const {
+    const double rt = t[0];
+    st->res[0] = rt;
+    if (!first) st->n_reliable += 1;
+    st->since = 0;
+    st->maxr = sqrt(rt);
+    if (rt <= st->rest) {
+      st->flag = 2;
+    } else if (st->iter >= st->maxiter) {
+      st->flag = 3;
+    } else {
+      // the true residual replaces the iterated one; the search direction is kept
+      st->beta = first ? 0.0 : rt / st->rtr;
+      st->rtr = rt;
+      st->flag = 0;
+    }
+  }
+};
+
+// p = r_lo + beta p after a reliable update (beta = 0 at the start: p = r)
+template <class S>
+__global__ __launch_bounds__(kBlock) void k_mix_prestart(SpinorPtr<S> p, SpinorPtr<S> r, int vh,
+                                                         const MixState *st) {
+  using V = typename S::V;
+  if (st->flag) return;
+  const float beta = (float)st->beta;
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= vh) return;
+  V pv[12], rv[12];
+  load_spinor<S>(rv, r, i, vh);
+  if (beta != 0.f) {
+    load_spinor<S>(pv, p, i, vh);
+  } else {
+#pragma unroll
+    for (int k = 0; k < 12; k++) pv[k] = cmk<V>(0.f, 0.f);
+  }
+#pragma unroll
+  for (int k = 0; k < 12; k++) {
+    pv[k].x = rv[k].x + beta * pv[k].x;
+    pv[k].y = rv[k].y + beta * pv[k].y;
+  }
+  store_spinor<S>(p, i, vh, pv);
+}
+
+// ---- low-precision copies of links and clover ------------------------------------
+template <class S> struct LP;
+template <> struct LP<SF32> {
+  static const float2 *gauge(Context *c) { return c->gauge32; }
+  static const float2 *clov(Context *c, int k) { return c->clov32[k]; }
+  static const float *norm(Context *, int) { return nullptr; }
+};
+template <> struct LP<SF16> {
+  static const short2 *gauge(Context *c) { return c->gauge16; }
+  static const short2 *clov(Context *c, int k) { return c->clov16[k]; }
+  static const float *norm(Context *c, int k) { return c->clov16n[k]; }
+};
+
+int ensure_lowprec(Context *c, int prec) {
+  const size_t vh = c->g.vh;
+  const bool clov = use_clover(c);
+  if (c->lp_gauge_version == c->gauge_version * 1000 + prec &&
+      c->lp_clover_version == c->clover_version * 1000 + prec)
+    return HB_OK;
+  const int grid = nblocks((int)std::min<size_t>(72 * vh, (size_t)1 << 22));
+  if (prec == 32) {
+    if (!c->gauge32) HB_HIP(hipMalloc(&c->gauge32, sizeof(float2) * 72 * vh));
+    k_cvt_field<SF32><<<grid, kBlock, 0, c->stream>>>(c->gauge32, c->gauge, 72 * vh);
+    HB_HIP(hipGetLastError());
+    if (clov) {
+      for (int k = 0; k < 2; k++) {
+        if (!c->clov32[k]) HB_HIP(hipMalloc(&c->clov32[k], sizeof(float2) * 72 * vh));
+        k_cvt_field<SF32><<<grid, kBlock, 0, c->stream>>>(c->clov32[k], c->clov[k], 72 * vh);
+        HB_HIP(hipGetLastError());
+      }
+    }
+  } else {
+    if (!c->gauge16) HB_HIP(hipMalloc(&c->gauge16, sizeof(short2) * 72 * vh));
+    k_cvt_field<SF16><<<grid, kBlock, 0, c->stream>>>(c->gauge16, c->gauge, 72 * vh);
+    HB_HIP(hipGetLastError());
+    if (clov) {
+      for (int k = 0; k < 2; k++) {
+        if (!c->clov16[k]) HB_HIP(hipMalloc(&c->clov16[k], sizeof(short2) * 72 * vh));
+        if (!c->clov16n[k]) HB_HIP(hipMalloc(&c->clov16n[k], sizeof(float) * 4 * vh));
+        k_cvt_clover16<<<dim3(nblocks((int)vh), 4), kBlock, 0, c->stream>>>(c->clov16[k], c->clov16n[k],
+                                                                          c->clov[k], (int)vh);
+        HB_HIP(hipGetLastError());
+      }
+    }
+  }
+  c->lp_gauge_version = c->gauge_version * 1000 + prec;
+  c->lp_clover_version = c->clover_version * 1000 + prec;
+  return HB_OK;
+}
+
+// half of M~ (or M~^+) with spinors in S and links/clover in G; see mtil_f64
+template <class S, class G>
+int mtil_lp(Context *c, bool dag, double kappa, double rho, SpinorPtr<S> out, SpinorPtr<S> in,
+            SpinorPtr<S> t, double *dot_out, SpinorPtr<S> dotv, const int *skip) {
+  const bool clov = use_clover(c);
+  DslashArgs<S, G> a1{};
+  a1.g = c->g;
+  a1.out_parity = 1;
+  a1.out = t;
+  a1.in = in;
+  a1.U = LP<G>::gauge(c);
+  a1.C = clov ? LP<G>::clov(c, HB_CLOVER_AINV) : nullptr;
+  a1.Cn = clov ? LP<G>::norm(c, HB_CLOVER_AINV) : nullptr;
+  a1.skip = skip;
+  HB_TRY(launch_dslash(a1, dag, EPI_CLINV, c->stream));
+  DslashArgs<S, G> a2{};
+  a2.g = c->g;
+  a2.out_parity = 0;
+  a2.out = out;
+  a2.in = t;
+  a2.U = LP<G>::gauge(c);
+  a2.C = clov ? LP<G>::clov(c, HB_CLOVER_A) : nullptr;
+  a2.Cn = clov ? LP<G>::norm(c, HB_CLOVER_A) : nullptr;
+  a2.x0 = in;
+  a2.mkappa2 = -kappa * kappa;
+  a2.rho = rho;
+  a2.skip = skip;
+  a2.partials = c->red;
+  if (dot_out) {
+    a2.dotv = dotv;
+    a2.dot_out = dot_out;
+    HB_TRY(launch_dslash(a2, dag, EPI_MTDOT, c->stream));
+  } else {
+    HB_TRY(launch_dslash(a2, dag, EPI_MTIL, c->stream));
+  }
+  return HB_OK;
+}
+
+template <class S> struct SpinorAlloc {
+  static SpinorPtr<S> get(Context *c, int slot, int nslot, int *rc) {
+    size_t vh = c->g.vh;
+    size_t bytes = sizeof(typename S::ST) * 12 * vh;
+    SpinorPtr<S> p{static_cast<typename S::ST *>(scratch(c, slot, bytes, rc)), nullptr};
+    if (*rc) return p;
+    if (S::kNorm) p.norm = static_cast<float *>(scratch(c, nslot, sizeof(float) * vh, rc));
+    return p;
+  }
+};
+
+template <class S, class G>
+int cg_mixed_impl(Context *c, int op, double kappa, double rho, double2 *x, const double2 *b,
+                  float rest, int maxiter, int log, double delta, int max_inner, int *iters,
+                  int *n_rel, double *true_rtr) {
+  const int vh = c->g.vh;
+  const double r_op = op == HB_OP_WDAGW ? rho : 0.0;
+  int rc = 0;
+  // FP64 work fields
+  double2 *t64 = spinor_scratch(c, SLOT_T, &rc); HB_TRY(rc);
+  double2 *y64 = spinor_scratch(c, SLOT_Y, &rc); HB_TRY(rc);
+  double2 *ax = spinor_scratch(c, SLOT_MIX0, &rc); HB_TRY(rc);
+  float2 *xlo = static_cast<float2 *>(scratch(c, SLOT_MIX1, sizeof(float2) * 12 * (size_t)vh, &rc)); HB_TRY(rc);
+  // low-precision fields (norm slots follow SLOT_LPN)
+  SpinorPtr<S> p = SpinorAlloc<S>::get(c, SLOT_LP0, SLOT_LPN, &rc); HB_TRY(rc);
+  SpinorPtr<S> r = SpinorAlloc<S>::get(c, SLOT_LP1, SLOT_LPN1, &rc); HB_TRY(rc);
+  SpinorPtr<S> q = SpinorAlloc<S>::get(c, SLOT_LP2, SLOT_LPN2, &rc); HB_TRY(rc);
+  SpinorPtr<S> t = SpinorAlloc<S>::get(c, SLOT_LP3, SLOT_LPN3, &rc); HB_TRY(rc);
+  SpinorPtr<S> y = SpinorAlloc<S>::get(c, SLOT_LP4, SLOT_LPN4, &rc); HB_TRY(rc);
+  MixState *st = static_cast<MixState *>(scratch(c, SLOT_STATE, sizeof(MixState), &rc)); HB_TRY(rc);
+  HB_TRY(ensure_host_stage(c, sizeof(MixState)));
+  MixState *hs = static_cast<MixState *>(c->host_stage);
+  HB_TRY(ensure_lowprec(c, G::kBytes == 8 ? 32 : 16));
+
+  hipEvent_t e0, e1;
+  HB_HIP(hipEventCreate(&e0));
+  HB_HIP(hipEventCreate(&e1));
+  HB_HIP(hipEventRecord(e0, c->stream));
+  MixState init{};
+  init.rest = (double)rest;
+  init.delta = delta;
+  init.max_inner = max_inner;
+  init.maxiter = maxiter;
+  *hs = init;
+  HB_HIP(hipMemcpyAsync(st, hs, sizeof(MixState), hipMemcpyHostToDevice, c->stream));
+  HB_TRY(zero_f64(c, reinterpret_cast<double2 *>(xlo), (size_t)6 * vh));
+
+  const int nvec = 12 * vh;
+  int nmv = 0;
+  auto reliable = [&](int first) -> int {
+    if (!first) {
+      k_mix_accum<<<nblocks(std::min(nvec, 1 << 22)), kBlock, 0, c->stream>>>(x, xlo, (size_t)nvec);
+      HB_HIP(hipGetLastError());
+    }
+    HB_TRY(apply_f64(c, op, kappa, rho, ax, x, t64, y64, nullptr, nullptr));
+    k_mix_true_residual<S><<<nblocks(vh), kBlock, 0, c->stream>>>(ax, b, r, vh, c->red);
+    k_finalize<1><<<1, 256, 0, c->stream>>>(c->red, nblocks(vh), nullptr, FinMixTrue{st, first});
+    HB_HIP(hipGetLastError());
+    k_mix_prestart<S><<<nblocks(vh), kBlock, 0, c->stream>>>(p, r, vh, st);
+    HB_HIP(hipGetLastError());
+    nmv++;
+    return HB_OK;
+  };
+  HB_TRY(reliable(1));
+  const int *skip = &st->flag;
+  int chunk = 8;
+  for (;;) {
+    for (int k = 0; k < chunk; k++) {
+      HB_TRY((mtil_lp<S, G>(c, false, kappa, r_op, y, p, t, nullptr, SpinorPtr<S>{}, skip)));
+      HB_TRY((mtil_lp<S, G>(c, true, kappa, r_op, q, y, t, &st->pap, p, skip)));
+      k_mix_update<S><<<nblocks(vh), kBlock, 0, c->stream>>>(xlo, r, p, q, vh, st, c->red);
+      k_finalize<1><<<1, 256, 0, c->stream>>>(c->red, nblocks(vh), &st->flag, FinMixUpdate{st});
+      HB_HIP(hipGetLastError());
+      k_mix_pupdate<S><<<nblocks(vh), kBlock, 0, c->stream>>>(p, r, vh, st);
+      HB_HIP(hipGetLastError());
+    }
+    HB_HIP(hipMemcpyAsync(hs, st, sizeof(MixState), hipMemcpyDeviceToHost, c->stream));
+    HB_HIP(hipStreamSynchronize(c->stream));
+    if (hs->flag == 2 || hs->flag == 3) break;
+    if (hs->flag == 1) {
+      HB_TRY(reliable(0));
+      HB_HIP(hipMemcpyAsync(hs, st, sizeof(MixState), hipMemcpyDeviceToHost, c->stream));
+      HB_HIP(hipStreamSynchronize(c->stream));
+      if (hs->flag == 2 || hs->flag == 3) break;
+      // next chunk length: about the iterations one reliable interval took
+      chunk = 8;
+    } else if (chunk < 32) {
+      chunk *= 2;
+    }
+  }
+  HB_HIP(hipEventRecord(e1, c->stream));
+  HB_HIP(hipStreamSynchronize(c->stream));
+  float ms = 0.f;
+  HB_HIP(hipEventElapsedTime(&ms, e0, e1));
+  HB_HIP(hipEventDestroy(e0));
+  HB_HIP(hipEventDestroy(e1));
+  c->last_solve_ms = ms;
+  c->last_n_matvec = nmv + hs->iter;
+  if (iters) *iters = hs->iter;
+  if (n_rel) *n_rel = hs->n_reliable;
+  if (true_rtr) *true_rtr = hs->res[0];
+  if (hs->flag != 2 && log != 2) {
+    set_error("cg_mixed(): no convergence; true rtr = %.17g", hs->res[0]);
+    return HB_E_NOCONV;
+  }
+  return HB_OK;
+}
+
+int cg_mixed(Context *c, int op, double kappa, double rho, double2 *x, const double2 *b, float rest,
+             int maxiter, int log, int prec, double delta, int max_inner, int *iters, int *n_rel,
+             double *true_rtr) {
+  HB_CHECK(op == HB_OP_MTDAGMT || op == HB_OP_WDAGW, HB_E_ARG,
+           "cg_mixed: operator must be HB_OP_MTDAGMT or HB_OP_WDAGW (got %d)", op);
+  HB_CHECK(delta > 0.0 && delta < 1.0, HB_E_ARG, "cg_mixed: delta must be in (0, 1)");
+  HB_CHECK(max_inner >= 1, HB_E_ARG, "cg_mixed: max_inner must be >= 1");
+  HB_CHECK(maxiter >= 1, HB_E_ARG, "cg_mixed: maxiter must be >= 1");
+  if (prec == HB_PREC_F64)
+    return cg_f64(c, op, kappa, rho, x, b, rest, maxiter, log, iters, true_rtr);
+  if (prec == HB_PREC_F32)
+    return cg_mixed_impl<SF32, SF32>(c, op, kappa, rho, x, b, rest, maxiter, log, delta, max_inner, iters, n_rel, true_rtr);
+  if (prec == HB_PREC_F16)
+    return cg_mixed_impl<SF16, SF16>(c, op, kappa, rho, x, b, rest, maxiter, log, delta, max_inner, iters, n_rel, true_rtr);
+  if (prec == HB_PREC_F16_GAUGE)
+    return cg_mixed_impl<SF32, SF16>(c, op, kappa, rho, x, b, rest, maxiter, log, delta, max_inner, iters, n_rel, true_rtr);
+  set_error("cg_mixed: precision %d (use 64, 32, 16 or 18)", prec);
+  return HB_E_ARG;
+}
+
+}  // namespace hb
+
+using namespace hb;
+
+extern "C" {
+
+int hb_cg_mixed_d(hb_context *ctx, int op, double kappa, double rho, double *x_d, const double *b_d,
+                  float rest, int maxiter, int log, int precision, double delta, int max_inner,
+                  int *iters, int *n_reliable, double *true_rtr) {
+  Context *c = reinterpret_cast<Context *>(ctx);
+  HB_TRY(check_ready(c));
+  HB_CHECK(x_d && b_d && x_d != b_d, HB_E_ARG, "hb_cg_mixed: bad pointers");
+  HB_HIP(hipSetDevice(c->device));
+  if (n_reliable) *n_reliable = 0;
+  return cg_mixed(c, op, kappa, rho, reinterpret_cast<double2 *>(x_d), reinterpret_cast<const double2 *>(b_d),
+                  rest, maxiter, log, precision, delta, max_inner, iters, n_reliable, true_rtr);
+}
+
+int hb_cg_mixed(hb_context *ctx, int op, double kappa, double rho, double *x, const double *b,
+                float rest, int maxiter, int log, int precision, double delta, int max_inner,
+                int *iters, int *n_reliable, double *true_rtr) {
+  Context *c = reinterpret_cast<Context *>(ctx);
+  HB_TRY(check_ready(c));
+  HB_CHECK(x && b, HB_E_ARG, "hb_cg_mixed: null pointer");
+  HB_HIP(hipSetDevice(c->device));
+  int rc = 0;
+  double2 *dx = spinor_scratch(c, SLOT_X, &rc); HB_TRY(rc);
+  double2 *db = spinor_scratch(c, SLOT_B, &rc); HB_TRY(rc);
+  HB_TRY(spinor_upload(c, dx, x));
+  HB_TRY(spinor_upload(c, db, b));
+  if (n_reliable) *n_reliable = 0;
+  int r = cg_mixed(c, op, kappa, rho, dx, db, rest, maxiter, log, precision, delta, max_inner,
+                   iters, n_reliable, true_rtr);
+  if (r != HB_OK && r != HB_E_NOCONV) return r;
+  HB_TRY(spinor_download(c, x, dx));
+  return r;
+}
+
+}  // extern "C"

@@ -23,7 +23,11 @@ enum {
   SLOT_LP2,
   SLOT_LP3,
   SLOT_LP4,
-  SLOT_LPN,        // FP16 norms
+  SLOT_LPN,        // FP16 norms of SLOT_LP0..LP4
+  SLOT_LPN1,
+  SLOT_LPN2,
+  SLOT_LPN3,
+  SLOT_LPN4,
   SLOT_MIX0,       // FP64 fields of the mixed solver
   SLOT_MIX1,
   SLOT_FORCE0,     // force temporaries
@@ -32,7 +36,7 @@ enum {
   SLOT_FORCE3,
   SLOT_FORCE4,
   SLOT_MOM,
-  SLOT_MULTI = 32  // multi-RHS fields start here
+  SLOT_MULTI = 64  // multi-RHS fields start here
 };
 
 bool use_clover(const Context *c);

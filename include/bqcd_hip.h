@@ -53,8 +53,10 @@ enum {
 /* clover fields */
 enum { HB_CLOVER_A = 0, HB_CLOVER_AINV = 1, HB_CLOVER_B = 2 };
 
-/* precisions of the inner solver */
-enum { HB_PREC_F64 = 64, HB_PREC_F32 = 32, HB_PREC_F16 = 16 };
+/* precisions of the inner solver: FP64; FP32; FP16 storage of spinors, links
+ * and clover (per-site scales, FP32 arithmetic); FP16 links and clover with
+ * FP32 spinors */
+enum { HB_PREC_F64 = 64, HB_PREC_F32 = 32, HB_PREC_F16 = 16, HB_PREC_F16_GAUGE = 18 };
 
 const char *hb_last_error(void);
 

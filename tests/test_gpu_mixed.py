@@ -1,0 +1,55 @@
+"""Mixed-precision CG (FP32 / FP16-storage inner solver, FP64 reliable updates):
+the stopping rule is the FP64 true residual, so |b - A x|^2 <= rest must hold
+exactly as for the FP64 solver; the solution must agree with the FP64 solve to
+the accuracy the residual bound implies (condition number x 1e-10)."""
+import numpy as np
+import pytest
+
+import oracle_lib as ol
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def setup():
+    import bqcd_hip
+    L = (8, 8, 8, 8)
+    lat = ol.Lattice(L)
+    u = lat.gauge_exp_random(321, 0.3)
+    kappa, csw = 0.13, 1.0
+    a, ai, b = lat.clover_init(u, csw * kappa)
+    phi = lat.spinor_gauss(322, 0)
+    g = bqcd_hip.Lattice(L)
+    g.load_gauge(u)
+    g.load_clover(a, ai, b)
+    yield g, lat, phi, kappa
+    g.close()
+
+
+@pytest.mark.parametrize("prec", ["f32", "f16"])
+@pytest.mark.parametrize("op,rho", [("mtdagmt", 0.0), ("w_dagger_w", 0.1)])
+def test_mixed_true_residual(setup, prec, op, rho):
+    g, lat, phi, kappa = setup
+    rest = 1e-20 * float(np.sum(phi * phi))
+    x64, it64, tr64 = g.cg(op, phi, kappa, rho, rest=rest, maxiter=5000)
+    x, it, nrel, tr = g.cg_mixed(op, phi, kappa, rho, rest=rest, maxiter=5000, precision=prec,
+                                 delta=0.01, max_inner=100)
+    assert tr <= rest            # FP64 true residual, the stopping rule itself
+    assert nrel >= 1
+    # recompute the true residual independently with the FP64 operator
+    r = phi - g.apply(op, x, kappa, rho)
+    assert abs(float(np.sum(r * r)) - tr) <= 1e-3 * tr + 1e-30
+    err = np.linalg.norm(x - x64) / np.linalg.norm(x64)
+    assert err < 1e-7, err
+    assert it <= 3 * it64
+
+
+def test_mixed_from_guess_and_maxiter(setup):
+    g, lat, phi, kappa = setup
+    rest = 1e-20 * float(np.sum(phi * phi))
+    x64, _, _ = g.cg("mtdagmt", phi, kappa, rest=rest, maxiter=5000)
+    x, it, nrel, tr = g.cg_mixed("mtdagmt", phi, kappa, rest=rest, x0=x64, precision="f16")
+    assert tr <= rest and it == 0 and nrel == 0
+    import bqcd_hip
+    with pytest.raises(bqcd_hip.NoConvergence):
+        g.cg_mixed("mtdagmt", phi, kappa, rest=1e-40, maxiter=20, precision="f16")
