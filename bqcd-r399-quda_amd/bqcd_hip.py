@@ -77,6 +77,13 @@ def lib():
         L.hb_cg_mixed.argtypes = [_vp, ctypes.c_int, ctypes.c_double, ctypes.c_double, _dp, _dp,
                                   ctypes.c_float, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                   ctypes.c_double, ctypes.c_int, _ip, _ip, _dp]
+        L.hb_cg_multi.argtypes = [_vp, ctypes.c_int, ctypes.c_int, _dp, _dp, ctypes.c_double,
+                                  ctypes.POINTER(_dp), ctypes.POINTER(_dp), ctypes.POINTER(ctypes.c_float),
+                                  ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, _ip, _dp]
+        L.hb_cg_multi_d.argtypes = [_vp, ctypes.c_int, ctypes.c_int, _dp, _dp, ctypes.c_double,
+                                    ctypes.POINTER(_vp), ctypes.POINTER(_vp), ctypes.POINTER(ctypes.c_float),
+                                    ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, _ip, _dp]
+        L.hb_multi_block_size.argtypes = [_vp]
         L.hb_cg_mixed_d.argtypes = [_vp, ctypes.c_int, ctypes.c_double, ctypes.c_double, _vp, _vp,
                                     ctypes.c_float, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                     ctypes.c_double, ctypes.c_int, _ip, _ip, _dp]
@@ -240,6 +247,25 @@ class Lattice:
                                  ctypes.byref(nrel), ctypes.byref(tr)))
         return x, it.value, nrel.value, tr.value
 
+    def cg_multi(self, op, bs, kappas, rhos=None, csw=0.0, rests=None, maxiter=20000, log=0,
+                 x0s=None, precision="f16", block=0):
+        """Multi-RHS solve; returns (xs, iters, true |r|^2 per RHS, block size used)."""
+        n = len(bs)
+        rhos = [0.0] * n if rhos is None else rhos
+        rests = [1e-20] * n if rests is None else rests
+        xs = [self.spinor() if x0s is None else np.array(x0s[k], dtype=np.float64, copy=True)
+              for k in range(n)]
+        bs = [np.ascontiguousarray(b, dtype=np.float64) for b in bs]
+        xp = (_dp * n)(*[_p(x) for x in xs])
+        bp = (_dp * n)(*[_p(b) for b in bs])
+        it = (ctypes.c_int * n)()
+        tr = (ctypes.c_double * n)()
+        _check(lib().hb_cg_multi(self._h, OP[op], n, (ctypes.c_double * n)(*kappas),
+                                 (ctypes.c_double * n)(*rhos), csw, xp, bp,
+                                 (ctypes.c_float * n)(*rests), maxiter, log, PREC[precision], block,
+                                 it, tr))
+        return xs, list(it), list(tr), lib().hb_multi_block_size(self._h)
+
     # ---- device-resident API (torch tensors, SoA layout) ---------------------
     def set_stream(self, stream=None):
         _check(lib().hb_set_stream(self._h, ctypes.c_void_p(stream.cuda_stream) if stream else None))
@@ -274,6 +300,21 @@ class Lattice:
                                    maxiter, log, PREC[precision], delta, max_inner, ctypes.byref(it),
                                    ctypes.byref(nrel), ctypes.byref(tr)))
         return it.value, nrel.value, tr.value
+
+    def cg_multi_d(self, op, xs, bs, kappas, rhos=None, csw=0.0, rests=None, maxiter=20000,
+                   log=0, precision="f16", block=0):
+        n = len(bs)
+        rhos = [0.0] * n if rhos is None else rhos
+        rests = [1e-20] * n if rests is None else rests
+        it = (ctypes.c_int * n)()
+        tr = (ctypes.c_double * n)()
+        _check(lib().hb_cg_multi_d(self._h, OP[op], n, (ctypes.c_double * n)(*kappas),
+                                   (ctypes.c_double * n)(*rhos), csw,
+                                   (_vp * n)(*[t.data_ptr() for t in xs]),
+                                   (_vp * n)(*[t.data_ptr() for t in bs]),
+                                   (ctypes.c_float * n)(*rests), maxiter, log, PREC[precision],
+                                   block, it, tr))
+        return list(it), list(tr), lib().hb_multi_block_size(self._h)
 
     def last_timing(self):
         ms = ctypes.c_double(0.0)

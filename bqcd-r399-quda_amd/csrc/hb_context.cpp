@@ -145,6 +145,11 @@ int hb_set_stream(hb_context *ctx, void *stream) {
   return HB_OK;
 }
 
+int hb_multi_block_size(hb_context *ctx) {
+  Context *c = reinterpret_cast<Context *>(ctx);
+  return c ? c->last_block : 0;
+}
+
 int hb_last_timing(hb_context *ctx, double *solve_ms, int *n_matvec) {
   Context *c = reinterpret_cast<Context *>(ctx);
   HB_CHECK(c, HB_E_ARG, "hb_last_timing: null context");

@@ -14,66 +14,9 @@
 //   EPI_MTIL   out = A x0 - kappa^2 (D in) + rho x0   (clover_mult_a + sc_xpby + sc_axpy)
 //   EPI_MTDOT  EPI_MTIL and a grid reduction of Re<v, out> (sc_dot), single pass
 #include "hb_dslash.h"
+#include "hb_hop.h"
 
 namespace hb {
-
-// One hop: acc += (1 + s gamma_mu) U psi(x +- mu), U = U_mu(x) forward or
-// U_mu(x - mu)^+ backward, with the boundary sign when the hop wraps.
-template <class S, class G, int MU, bool FWD, int SG>
-__device__ __forceinline__ void hop(typename S::V (&acc)[12], const DslashArgs<S, G> &a,
-                                    const SpinorView<S> &in, const Geom &g, int e, int i,
-                                    const int (&xc)[4]) {
-  using V = typename S::V;
-  using A = typename S::A;
-  const int vh = g.vh;
-  int c[4] = {xc[0], xc[1], xc[2], xc[3]};
-  A sgn = (A)1;
-  if (FWD) {
-    c[MU] += 1;
-    if (c[MU] == g.L[MU]) { c[MU] = 0; sgn = (A)g.bc[MU]; }
-  } else {
-    c[MU] -= 1;
-    if (c[MU] < 0) { c[MU] = g.L[MU] - 1; sgn = (A)g.bc[MU]; }
-  }
-  const int j = site_index(g, c[0], c[1], c[2], c[3]);
-  V h[6], w[6];
-  if constexpr (MU == 3) {
-    V p[6];
-    in.load_pair(p, j, SG < 0 ? 2 : 0);
-    const A f = (A)2 * sgn;
-#pragma unroll
-    for (int cc = 0; cc < 3; cc++) {
-      h[2 * cc] = cscale<V>(f, p[cc]);
-      h[2 * cc + 1] = cscale<V>(f, p[3 + cc]);
-    }
-  } else {
-    V p[12];
-    in.load(p, j);
-    project<MU, SG>(h, p);
-    if (sgn < (A)0) {
-#pragma unroll
-      for (int k = 0; k < 6; k++) h[k] = cmk<V>(-h[k].x, -h[k].y);
-    }
-  }
-  V u[9];
-  if (FWD) {
-    load_link<G>(u, a.U, MU, e, i, vh);
-    su3_mul_half(w, u, h);
-  } else {
-    load_link<G>(u, a.U, MU, 1 - e, j, vh);
-    su3_adj_mul_half(w, u, h);
-  }
-  reconstruct<MU, SG>(acc, w);
-}
-
-// Completes one hop before the next starts: the accumulator is pinned in
-// registers (so its arithmetic cannot be sunk to the final store) and the
-// scheduler may not hoist the next hop's loads above this point.
-template <class V> __device__ __forceinline__ void pin(V (&acc)[12]) {
-#pragma unroll
-  for (int k = 0; k < 12; k++) asm volatile("" : "+v"(acc[k].x), "+v"(acc[k].y));
-}
-#define HB_SCHED_FENCE() do { pin(acc); __builtin_amdgcn_sched_barrier(0); } while (0)
 
 template <class S, class G, bool DAG, int EPI>
 __global__ __launch_bounds__(kBlock) void k_dslash(DslashArgs<S, G> a) {
@@ -95,14 +38,14 @@ __global__ __launch_bounds__(kBlock) void k_dslash(DslashArgs<S, G> a) {
     // forward projector sign: (1 - gamma) for D, (1 + gamma) for D^+
     constexpr int SF = DAG ? 1 : -1;
     constexpr int SB = -SF;
-    hop<S, G, 3, true, SF>(acc, a, in, g, e, i, xc);  HB_SCHED_FENCE();
-    hop<S, G, 3, false, SB>(acc, a, in, g, e, i, xc); HB_SCHED_FENCE();
-    hop<S, G, 2, true, SF>(acc, a, in, g, e, i, xc);  HB_SCHED_FENCE();
-    hop<S, G, 2, false, SB>(acc, a, in, g, e, i, xc); HB_SCHED_FENCE();
-    hop<S, G, 1, true, SF>(acc, a, in, g, e, i, xc);  HB_SCHED_FENCE();
-    hop<S, G, 1, false, SB>(acc, a, in, g, e, i, xc); HB_SCHED_FENCE();
-    hop<S, G, 0, true, SF>(acc, a, in, g, e, i, xc);  HB_SCHED_FENCE();
-    hop<S, G, 0, false, SB>(acc, a, in, g, e, i, xc); HB_SCHED_FENCE();
+    hop<S, G, 3, true, SF>(acc, a.U, in, g, e, i, xc);  HB_SCHED_FENCE();
+    hop<S, G, 3, false, SB>(acc, a.U, in, g, e, i, xc); HB_SCHED_FENCE();
+    hop<S, G, 2, true, SF>(acc, a.U, in, g, e, i, xc);  HB_SCHED_FENCE();
+    hop<S, G, 2, false, SB>(acc, a.U, in, g, e, i, xc); HB_SCHED_FENCE();
+    hop<S, G, 1, true, SF>(acc, a.U, in, g, e, i, xc);  HB_SCHED_FENCE();
+    hop<S, G, 1, false, SB>(acc, a.U, in, g, e, i, xc); HB_SCHED_FENCE();
+    hop<S, G, 0, true, SF>(acc, a.U, in, g, e, i, xc);  HB_SCHED_FENCE();
+    hop<S, G, 0, false, SB>(acc, a.U, in, g, e, i, xc); HB_SCHED_FENCE();
   }
 
   if constexpr (EPI == EPI_PLAIN) {

@@ -257,6 +257,15 @@ __global__ __launch_bounds__(128) void k_ape(double2 *__restrict__ Unew, const d
   store_link(Unew, mu, e, i, vh, u);
 }
 
+int build_clover_into(Context *c, double csw_kappa, double2 *Ca, double2 *Ci, double2 *Cb) {
+  for (int e = 0; e < 2; e++) {
+    k_clover_build<<<nblocks(c->g.vh, 128), 128, 0, c->stream>>>(c->gauge, c->g, e, -csw_kappa / 8.0,
+                                                                 Ca, Ci, Cb);
+    HB_HIP(hipGetLastError());
+  }
+  return HB_OK;
+}
+
 int build_clover_dev(Context *c, double csw_kappa) {
   HB_TRY(alloc_fp64_fields(c));
   if (csw_kappa == 0.0) {
@@ -264,12 +273,8 @@ int build_clover_dev(Context *c, double csw_kappa) {
     c->clover_version++;
     return HB_OK;
   }
-  for (int e = 0; e < 2; e++) {
-    k_clover_build<<<nblocks(c->g.vh, 128), 128, 0, c->stream>>>(
-        c->gauge, c->g, e, -csw_kappa / 8.0, c->clov[HB_CLOVER_A], c->clov[HB_CLOVER_AINV],
-        c->clov[HB_CLOVER_B]);
-    HB_HIP(hipGetLastError());
-  }
+  HB_TRY(build_clover_into(c, csw_kappa, c->clov[HB_CLOVER_A], c->clov[HB_CLOVER_AINV],
+                           c->clov[HB_CLOVER_B]));
   for (int k = 0; k < 3; k++) c->have_clover[k] = true;
   c->clover_version++;
   return HB_OK;

@@ -117,6 +117,7 @@ struct Context {
   size_t dev_stage_bytes = 0;
   double last_solve_ms = 0.0;
   int last_n_matvec = 0;
+  int last_block = 0;                // RHS block size of the last multi-RHS solve
 };
 
 int ensure_dev_stage(Context *c, size_t bytes);

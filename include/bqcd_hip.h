@@ -117,14 +117,23 @@ int hb_cg_mixed(hb_context *ctx, int op, double kappa, double rho, double *x, co
                 float rest, int maxiter, int log, int precision, double delta, int max_inner,
                 int *iters, int *n_reliable, double *true_rtr);
 
-/* Multi right-hand-side solve: nrhs systems (op with kappa[k], rho[k]) with
- * host spinors x[k], b[k] (BQCD layout, each 24*volh doubles).  Right-hand
- * sides sharing (kappa, clover) are processed by one block hopping matrix
- * that reads every link once for the whole block; converged systems are
- * masked out.  rest[k] as hb_cg. block = RHS block size (0 = autotuned). */
+/* Multi right-hand-side solve (no counterpart in the reference, which solves
+ * one system per cg() call): nrhs systems op(kappa[k], rho[k]) x[k] = b[k]
+ * with host spinors in BQCD layout.  A block of right-hand sides shares one
+ * pass of the block hopping matrix (every link and clover word read once for
+ * the block); systems with different kappa (Hasenbusch shifts) can share a
+ * block; converged systems are masked out.  csw > 0 builds the clover term of
+ * every distinct kappa on the device from the loaded links, csw == 0 means no
+ * clover term, csw < 0 uses the loaded clover field (one kappa only).  Inner
+ * iterations in precision (HB_PREC_F32 or HB_PREC_F16), FP64 reliable updates
+ * per system; each stops on its FP64 true residual <= rest[k].  block = RHS
+ * block size 1, 2, 4, 8 or 12, or 0 to let the autotuner time the candidates
+ * and choose.  iters[k] and true_rtr[k] are outputs. */
 int hb_cg_multi(hb_context *ctx, int op, int nrhs, const double *kappa, const double *rho,
-                double *const *x, const double *const *b, const float *rest, int maxiter,
-                int log, int precision, int block, int *iters, double *true_rtr);
+                double csw, double *const *x, const double *const *b, const float *rest,
+                int maxiter, int log, int precision, int block, int *iters, double *true_rtr);
+/* block size chosen by the last hb_cg_multi call */
+int hb_multi_block_size(hb_context *ctx);
 
 /* ---- forces (host, BQCD layout; p accumulates) -------------------------- */
 /* dsf (dsf.F90:28-110): hopping + clover part of dS_f/dU for spinors a, b
@@ -153,8 +162,8 @@ int hb_cg_mixed_d(hb_context *ctx, int op, double kappa, double rho, double *x_d
                   const double *b_d, float rest, int maxiter, int log, int precision,
                   double delta, int max_inner, int *iters, int *n_reliable, double *true_rtr);
 int hb_cg_multi_d(hb_context *ctx, int op, int nrhs, const double *kappa, const double *rho,
-                  double *const *x_d, const double *const *b_d, const float *rest, int maxiter,
-                  int log, int precision, int block, int *iters, double *true_rtr);
+                  double csw, double *const *x_d, const double *const *b_d, const float *rest,
+                  int maxiter, int log, int precision, int block, int *iters, double *true_rtr);
 /* BQCD layout <-> SoA conversions of one spinor (host <-> device) */
 int hb_spinor_to_device(hb_context *ctx, double *dst_d, const double *src_host);
 int hb_spinor_to_host(hb_context *ctx, double *dst_host, const double *src_d);

@@ -1,0 +1,73 @@
+"""Multi-RHS solver: every block size and the autotuner, right-hand sides with
+different kappa (Hasenbusch-style shifts) in one block, W^+W with rho.  Each
+system must reach its FP64 true residual bound and agree with an independent
+single-RHS FP64 solve (clover term built by the oracle for its kappa)."""
+import numpy as np
+import pytest
+
+import oracle_lib as ol
+
+pytestmark = pytest.mark.gpu
+CSW = 1.9
+
+
+@pytest.fixture(scope="module")
+def setup():
+    import bqcd_hip
+    L = (4, 4, 4, 8)
+    lat = ol.Lattice(L)
+    u = lat.gauge_exp_random(555, 0.35)
+    kappas = [0.125, 0.12, 0.11]
+    refs = {}
+    for k in kappas:
+        a, ai, b = lat.clover_init(u, CSW * k)
+        refs[k] = lat.op(lat.flip_bc(u), a, ai, k, CSW * k, 0.05)
+    phis = [lat.spinor_gauss(900 + j, 0) for j in range(12)]
+    g = bqcd_hip.Lattice(L)
+    g.load_gauge(u)
+    yield g, lat, refs, phis, kappas
+    g.close()
+
+
+def check(lat, refs, xs, its, trs, phis, kap, rests, which, rhos=None):
+    for k, x in enumerate(xs):
+        assert trs[k] <= rests[k], (k, trs[k], rests[k])
+        assert its[k] > 0
+        xr, _, _ = lat.cg(refs[kap[k]], which, phis[k], rests[k] * 1e-4, 5000)
+        err = np.linalg.norm(x - xr) / np.linalg.norm(xr)
+        assert err < 1e-7, (k, err)
+
+
+@pytest.mark.parametrize("block", [1, 2, 4, 8, 12])
+def test_multi_block_sizes_mixed_kappa(setup, block):
+    g, lat, refs, phis, kappas = setup
+    kap = [kappas[j % 3] for j in range(12)]
+    rests = [1e-20 * float(np.sum(p * p)) for p in phis]
+    xs, its, trs, used = g.cg_multi("mtdagmt", phis, kap, csw=CSW, rests=rests, block=block,
+                                    precision="f16")
+    assert used == block
+    check(lat, refs, xs, its, trs, phis, kap, rests, 2)
+    # uneven convergence: lighter kappa needs more iterations
+    assert max(its[k] for k in range(12) if kap[k] == 0.125) > max(its[k] for k in range(12) if kap[k] == 0.11)
+
+
+@pytest.mark.parametrize("prec", ["f32", "f16"])
+def test_multi_autotune_wdw(setup, prec):
+    g, lat, refs, phis, kappas = setup
+    kap = [kappas[0]] * 5
+    rests = [1e-20 * float(np.sum(p * p)) for p in phis[:5]]
+    xs, its, trs, used = g.cg_multi("w_dagger_w", phis[:5], kap, rhos=[0.05] * 5, csw=CSW,
+                                    rests=rests, block=0, precision=prec)
+    assert used in (1, 2, 4, 8, 12)
+    check(lat, refs, xs, its, trs, phis, kap, rests, 5)
+
+
+def test_multi_with_loaded_clover_single_kappa(setup):
+    g, lat, refs, phis, kappas = setup
+    u = g.save_gauge()
+    a, ai, b = lat.clover_init(u, CSW * kappas[1])
+    g.load_clover(a, ai, b)
+    kap = [kappas[1]] * 3
+    rests = [1e-20 * float(np.sum(p * p)) for p in phis[:3]]
+    xs, its, trs, used = g.cg_multi("mtdagmt", phis[:3], kap, csw=-1.0, rests=rests, block=2)
+    check(lat, refs, xs, its, trs, phis, kap, rests, 2)
