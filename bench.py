@@ -48,6 +48,9 @@ def parse():
     ap.add_argument("--dslash-reps", type=int, default=50)
     ap.add_argument("--cpu-iters", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-batch", action="store_true",
+                    help="skip the multi-RHS workload (24^3x48, 12 RHS over 3 kappa)")
+    ap.add_argument("--batch-lattice", default="24,24,24,48")
     ap.add_argument("--seed", type=int, default=20261202)
     ap.add_argument("--sweeps", type=int, default=5)
     ap.add_argument("--alpha", type=float, default=0.36)
@@ -116,6 +119,81 @@ def cpu_baseline(g, b_dev, kappa, csw, gpu_iters, cpu_iters):
     return {"value": dt / cpu_iters * gpu_iters, "unit": "s", "cores": 1, "kind": "port",
             "sample": "oracle C restatement cg(), %d iterations in %.2f s, scaled to %d iterations"
                       % (cpu_iters, dt, gpu_iters)}
+
+
+def batched(args, torch, dev, stream):
+    """Multi-RHS workload: 12 right-hand sides over 3 Hasenbusch kappa values
+    (0.1355, 0.13, 0.12; 4 Gaussian sources each), csw 1.9, tolerance
+    |r|/|b| <= 1e-10 each; the block solver (autotuned RHS block) against 12
+    sequential single-RHS solves with the same mixed-precision solver and with
+    the FP64 solver."""
+    import bqcd_hip
+
+    L = tuple(int(v) for v in args.batch_lattice.split(","))
+    kappas = [0.1355] * 4 + [0.13] * 4 + [0.12] * 4
+    g = bqcd_hip.Lattice(L)
+    g.set_stream(stream)
+    g.gauge_random(args.seed + 7, 0.0)
+    g.gauge_smear(args.sweeps, args.alpha)
+    plaq = g.plaquette()
+    gen = torch.Generator(device=dev)
+    gen.manual_seed(args.seed + 8)
+    bs = [torch.randn((12, g.volh, 2), dtype=torch.float64, device=dev, generator=gen) * (0.5 ** 0.5)
+          for _ in kappas]
+    xs = [torch.zeros_like(b) for b in bs]
+    rests = [1e-20 * float((b * b).sum()) for b in bs]
+    prec = "f16" if args.solver != "mixed32" else "f32"
+
+    def timed(fn):
+        torch.cuda.synchronize()
+        t0 = torch.cuda.Event(enable_timing=True)
+        t1 = torch.cuda.Event(enable_timing=True)
+        t0.record(stream)
+        out = fn()
+        t1.record(stream)
+        torch.cuda.synchronize()
+        return t0.elapsed_time(t1) / 1e3, out
+
+    def sequential(mixed):
+        its = []
+        for k, kap in enumerate(kappas):
+            if k == 0 or kap != kappas[k - 1]:
+                g.build_clover(args.csw * kap)
+            xs[k].zero_()
+            if mixed:
+                it, _, tr = g.cg_mixed_d("mtdagmt", xs[k], bs[k], kap, rest=rests[k], maxiter=40000,
+                                         precision=prec, delta=args.delta, max_inner=args.max_inner)
+            else:
+                it, tr = g.cg_d("mtdagmt", xs[k], bs[k], kap, rest=rests[k], maxiter=40000)
+            its.append(it)
+        return its
+
+    def block_solve():
+        for x in xs:
+            x.zero_()
+        return g.cg_multi_d("mtdagmt", xs, bs, kappas, csw=args.csw, rests=rests, maxiter=40000,
+                            precision=prec, block=0)
+
+    block_solve()  # autotunes the RHS block size (cached per lattice and precision)
+    t_blk, (its_blk, trs_blk, blk) = timed(block_solve)
+    x_blk = [x.clone() for x in xs]
+    t_seq_mixed, its_seq = timed(lambda: sequential(True))
+    t_seq_f64, its_f64 = timed(lambda: sequential(False))
+    err = max(float(torch.linalg.vector_norm(x_blk[k] - xs[k]) / torch.linalg.vector_norm(xs[k]))
+              for k in range(len(kappas)))
+    res = {
+        "lattice": list(L), "plaquette": round(plaq, 6), "nrhs": len(kappas), "kappas": kappas,
+        "csw": args.csw, "precision": prec, "block_size": blk,
+        "block_solve_s": t_blk, "sequential_mixed_s": t_seq_mixed, "sequential_fp64_s": t_seq_f64,
+        "speedup_vs_sequential_mixed": t_seq_mixed / t_blk,
+        "speedup_vs_sequential_fp64": t_seq_f64 / t_blk,
+        "iterations_block": its_blk, "iterations_sequential_fp64": its_f64,
+        "max_true_residual_rel": max(trs_blk[k] / (rests[k] * 1e20) for k in range(len(kappas))),
+        "max_solution_rel_error_vs_fp64": err,
+    }
+    log("batched:", json.dumps(res))
+    g.close()
+    return res
 
 
 def scan(g, args, torch, dev):
@@ -269,6 +347,11 @@ def main():
         except Exception as exc:  # reported, never silently replaced
             log("cpu baseline failed:", exc)
 
+    batch = None
+    if not args.no_batch:
+        g.close()
+        batch = batched(args, torch, dev, stream)
+
     cg_bytes = 10560 * vh * (fp64.get("iters") or iters[0])
     fp64_ms = fp64.get("ms", ms)
     line = {
@@ -309,6 +392,7 @@ def main():
             "bytes_per_launch": ds_bytes,
         },
         "cpu_baseline": cpu,
+        "batched": batch,
     }
     print(json.dumps(line), flush=True)
     g.close()

@@ -36,6 +36,38 @@ class HipError(RuntimeError):
     pass
 
 
+class HmcParams(ctypes.Structure):
+    """hb_hmc_params of include/bqcd_hip.h"""
+    _fields_ = [("beta", ctypes.c_double), ("kappa", ctypes.c_double), ("csw", ctypes.c_double),
+                ("rho", ctypes.c_double), ("tau", ctypes.c_double), ("ntau", ctypes.c_int),
+                ("m_scale", ctypes.c_int), ("model", ctypes.c_char), ("integrator", ctypes.c_int),
+                ("lam", ctypes.c_double), ("rest", ctypes.c_float), ("maxiter", ctypes.c_int),
+                ("precision", ctypes.c_int), ("delta", ctypes.c_double), ("max_inner", ctypes.c_int),
+                ("mre_vectors", ctypes.c_int), ("batch", ctypes.c_int)]
+
+
+class HmcOut(ctypes.Structure):
+    """hb_hmc_out of include/bqcd_hip.h"""
+    _fields_ = [(n, ctypes.c_double) for n in (
+        "exp_dh", "h_old", "h_new", "sg_old", "sg_new", "sp_old", "sp_new", "sd_old", "sd_new",
+        "sf1_old", "sf1_new", "sf2_old", "sf2_new", "plaquette")] + \
+        [(n, ctypes.c_int) for n in ("accepted", "cg_ncall", "cg_niter_tot", "cg_niter_max",
+                                     "n_force_ir", "n_force_uv")] + [("seconds", ctypes.c_double)]
+
+    def as_dict(self):
+        return {n: getattr(self, n) for n, _ in self._fields_}
+
+
+INTEGRATOR = {"leapfrog": 0, "omelyan": 1}
+
+
+def hmc_params(beta, kappa, csw, rho, tau, ntau, m_scale=1, model="C", integrator="leapfrog",
+               lam=0.0, rest=1e-20, maxiter=10000, precision="f64", delta=0.1, max_inner=1000,
+               mre_vectors=0, batch=False):
+    return HmcParams(beta, kappa, csw, rho, tau, ntau, m_scale, model.encode(), INTEGRATOR[integrator],
+                     lam, rest, maxiter, PREC[precision], delta, max_inner, mre_vectors, int(batch))
+
+
 class NoConvergence(HipError):
     pass
 
@@ -70,6 +102,11 @@ def lib():
         L.hb_spinor_to_device.argtypes = [_vp, _vp, _dp]
         L.hb_spinor_to_host.argtypes = [_vp, _dp, _vp]
         L.hb_last_timing.argtypes = [_vp, _dp, _ip]
+        L.hb_hmc_trajectory.argtypes = [_vp, ctypes.POINTER(HmcParams), _dp, ctypes.c_int, _dp, _dp,
+                                        ctypes.c_double, ctypes.POINTER(HmcOut)]
+        L.hb_hmc_trajectory_d.argtypes = [_vp, ctypes.POINTER(HmcParams), _vp, _vp, _vp, ctypes.c_double,
+                                          ctypes.POINTER(HmcOut)]
+        L.hb_gaussian_d.argtypes = [_vp, ctypes.c_ulonglong, ctypes.c_double, _vp, ctypes.c_longlong]
         L.hb_build_clover.argtypes = [_vp, ctypes.c_double]
         L.hb_plaquette.argtypes = [_vp, _dp]
         L.hb_gauge_random.argtypes = [_vp, ctypes.c_ulonglong, ctypes.c_double]
@@ -84,6 +121,17 @@ def lib():
                                     ctypes.POINTER(_vp), ctypes.POINTER(_vp), ctypes.POINTER(ctypes.c_float),
                                     ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, _ip, _dp]
         L.hb_multi_block_size.argtypes = [_vp]
+        L.hb_dsf.argtypes = [_vp, _dp, ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_double, _dp, _dp]
+        L.hb_dsd.argtypes = [_vp, _dp, ctypes.c_int, ctypes.c_double, ctypes.c_double]
+        L.hb_dsg.argtypes = [_vp, _dp, ctypes.c_int, ctypes.c_double, ctypes.c_double]
+        L.hb_hmc_u.argtypes = [_vp, _dp, ctypes.c_int, ctypes.c_double, ctypes.c_double]
+        L.hb_clover_action.argtypes = [_vp, _dp]
+        L.hb_dsf_d.argtypes = [_vp, _vp, ctypes.c_double, ctypes.c_double, ctypes.c_double, _vp, _vp]
+        L.hb_dsd_d.argtypes = [_vp, _vp, ctypes.c_double, ctypes.c_double]
+        L.hb_dsg_d.argtypes = [_vp, _vp, ctypes.c_double, ctypes.c_double]
+        L.hb_hmc_u_d.argtypes = [_vp, _vp, ctypes.c_double, ctypes.c_double]
+        L.hb_gauge_backup.argtypes = [_vp, ctypes.c_int]
+        L.hb_gauge_restore.argtypes = [_vp, ctypes.c_int]
         L.hb_cg_mixed_d.argtypes = [_vp, ctypes.c_int, ctypes.c_double, ctypes.c_double, _vp, _vp,
                                     ctypes.c_float, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                     ctypes.c_double, ctypes.c_int, _ip, _ip, _dp]
@@ -167,6 +215,12 @@ class Lattice:
         s = ctypes.c_double(0.0)
         _check(lib().hb_plaquette(self._h, ctypes.byref(s)))
         return s.value / (3.0 * 6.0 * 2 * self.volh)
+
+    def plaquette_sum(self):
+        """sum over all plaquettes of Re Tr U_p"""
+        s = ctypes.c_double(0.0)
+        _check(lib().hb_plaquette(self._h, ctypes.byref(s)))
+        return s.value
 
     def gauge_random(self, seed, scale=0.0):
         """seeded synthetic links: exp(i scale w.lambda) for scale > 0, Haar SU(3) otherwise"""
@@ -266,6 +320,47 @@ class Lattice:
                                  it, tr))
         return xs, list(it), list(tr), lib().hb_multi_block_size(self._h)
 
+    # ---- forces and link update (host arrays) ----------------------------------
+    def gen(self, stride=None):
+        return np.zeros((4, 2, stride or self.volh, 8))
+
+    def dsf(self, a, b, step, kappa, csw_kappa, p=None):
+        """dsf (dsf.F90:28): p += hopping + clover fermion force for spinors a, b"""
+        p = self.gen() if p is None else np.array(p, dtype=np.float64, copy=True)
+        _check(lib().hb_dsf(self._h, _p(p), p.shape[2], step, kappa, csw_kappa,
+                            _p(np.ascontiguousarray(a, dtype=np.float64)),
+                            _p(np.ascontiguousarray(b, dtype=np.float64))))
+        return p
+
+    def dsd(self, step, csw_kappa, p=None):
+        """dsd (dsd.F90:28): p += clover-determinant force"""
+        p = self.gen() if p is None else np.array(p, dtype=np.float64, copy=True)
+        _check(lib().hb_dsd(self._h, _p(p), p.shape[2], step, csw_kappa))
+        return p
+
+    def dsg(self, step, beta, p=None):
+        """dsg (dsg.F90:28): p += Wilson gauge force"""
+        p = self.gen() if p is None else np.array(p, dtype=np.float64, copy=True)
+        _check(lib().hb_dsg(self._h, _p(p), p.shape[2], step, beta))
+        return p
+
+    def hmc_u(self, p, step, csw_kappa=0.0):
+        """hmc_u (hmc_u.F90:28): U <- exp(i step p.lambda) U, reunitarised, clover rebuilt"""
+        p = np.ascontiguousarray(p, dtype=np.float64)
+        _check(lib().hb_hmc_u(self._h, _p(p), p.shape[2], step, csw_kappa))
+
+    def clover_action(self):
+        """clover_action (clover/clover_action.F90:29) of the odd sites"""
+        s = ctypes.c_double(0.0)
+        _check(lib().hb_clover_action(self._h, ctypes.byref(s)))
+        return s.value
+
+    def gauge_backup(self, slot=0):
+        _check(lib().hb_gauge_backup(self._h, slot))
+
+    def gauge_restore(self, slot=0):
+        _check(lib().hb_gauge_restore(self._h, slot))
+
     # ---- device-resident API (torch tensors, SoA layout) ---------------------
     def set_stream(self, stream=None):
         _check(lib().hb_set_stream(self._h, ctypes.c_void_p(stream.cuda_stream) if stream else None))
@@ -315,6 +410,42 @@ class Lattice:
                                    (ctypes.c_float * n)(*rests), maxiter, log, PREC[precision],
                                    block, it, tr))
         return list(it), list(tr), lib().hb_multi_block_size(self._h)
+
+    def dsf_d(self, p, a, b, step, kappa, csw_kappa):
+        """p (4,2,8,volh) float64 device tensor accumulates the fermion force"""
+        _check(lib().hb_dsf_d(self._h, _tp(p), step, kappa, csw_kappa, _tp(a), _tp(b)))
+
+    def dsd_d(self, p, step, csw_kappa):
+        _check(lib().hb_dsd_d(self._h, _tp(p), step, csw_kappa))
+
+    def dsg_d(self, p, step, beta):
+        _check(lib().hb_dsg_d(self._h, _tp(p), step, beta))
+
+    def hmc_u_d(self, p, step, csw_kappa=0.0):
+        _check(lib().hb_hmc_u_d(self._h, _tp(p), step, csw_kappa))
+
+    # ---- HMC trajectory -------------------------------------------------------
+    def hmc_trajectory(self, prm, p, eta1, eta2=None, accept_random=-1.0):
+        """hmc() (hmc.F90:28) on host draws in BQCD layouts; returns hb_hmc_out as a dict"""
+        p = np.ascontiguousarray(p, dtype=np.float64)
+        out = HmcOut()
+        _check(lib().hb_hmc_trajectory(self._h, ctypes.byref(prm), _p(p), p.shape[2],
+                                       _p(np.ascontiguousarray(eta1, dtype=np.float64)),
+                                       None if eta2 is None else _p(np.ascontiguousarray(eta2, dtype=np.float64)),
+                                       accept_random, ctypes.byref(out)))
+        return out.as_dict()
+
+    def hmc_trajectory_d(self, prm, p, eta1, eta2=None, accept_random=-1.0):
+        out = HmcOut()
+        _check(lib().hb_hmc_trajectory_d(self._h, ctypes.byref(prm), _tp(p), _tp(eta1),
+                                         None if eta2 is None else _tp(eta2), accept_random,
+                                         ctypes.byref(out)))
+        return out.as_dict()
+
+    def gaussian_d(self, t, seed, var):
+        """fill the float64 device tensor t with N(0, var) numbers"""
+        _check(lib().hb_gaussian_d(self._h, seed, var, _tp(t), t.numel()))
+        return t
 
     def last_timing(self):
         ms = ctypes.c_double(0.0)

@@ -20,7 +20,8 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
 
 SOURCES = ["hb_context.cpp", "hb_fields.hip", "hb_dslash.hip", "hb_blas.hip", "hb_cg.hip",
-           "hb_gauge.hip", "hb_mixed.hip", "hb_multi.hip"]
+           "hb_gauge.hip", "hb_mixed.hip", "hb_multi.hip",
+           "hb_force.hip", "hb_hmc.hip"]
 
 FLAGS = [
     "-O3", "-std=c++17", "-fPIC", "--offload-arch=" + ARCH,

@@ -118,6 +118,11 @@ struct Context {
   double last_solve_ms = 0.0;
   int last_n_matvec = 0;
   int last_block = 0;                // RHS block size of the last multi-RHS solve
+  // chronological initial guesses (mre.F90) of the two pseudofermion solves
+  // of the HMC: basis vectors live in scratch slots, newest first
+  int mre_nvec = 0;
+  int mre_rank[2] = {0, 0};
+  int mre_slot[2][16] = {};
 };
 
 int ensure_dev_stage(Context *c, size_t bytes);

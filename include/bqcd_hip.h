@@ -74,10 +74,10 @@ int hb_volh(const hb_context *ctx);
  * keeps physical links and applies the boundary sign in its kernels. */
 int hb_load_gauge(hb_context *ctx, const double *u, int site_stride, int links_are_bc_flipped);
 int hb_save_gauge(hb_context *ctx, double *u, int site_stride, int bc_flipped);
-/* Uploads conf%a, conf%i, conf%b (any may be NULL). rho_shift is added to
- * the diagonal of A (folded as +rho/2 on the stored A/2, as clover_rho,
- * invert_quda.F90:70-93); pass 0 and use the rho argument of the operators
- * instead when the clover field is shared between W and M~ solves. */
+/* Uploads conf%a, conf%i, conf%b (any may be NULL) as clover_init left them
+ * (A/2, A^-1/2, LDL^+ factors; replaces load_clover_quda,
+ * invert_quda.F90:70-93).  The rho shift of W is not folded into A: it is the
+ * rho argument of the operators, so one clover field serves W and M~. */
 int hb_load_clover(hb_context *ctx, const double *a, const double *ainv, const double *b);
 /* clover_init (clover/clover_init.F90:28-78) on the device from the loaded
  * physical links: builds A/2, A^-1/2 and the LDL^+ factors for both parities */
@@ -149,6 +149,10 @@ int hb_dsg(hb_context *ctx, double *p, int site_stride, double step, double beta
 int hb_hmc_u(hb_context *ctx, const double *p, int site_stride, double step, double csw_kappa);
 /* sum over plaquettes of Re Tr U_p (action.F90:70-123 uses it for S_g) */
 int hb_plaquette(hb_context *ctx, double *re_tr_sum);
+/* device copies of the links (up to 4 slots), e.g. the HMC backup before a
+ * trajectory (hmc.F90 keeps conf_old) */
+int hb_gauge_backup(hb_context *ctx, int slot);
+int hb_gauge_restore(hb_context *ctx, int slot);
 
 /* ---- device-pointer entry points (library SoA layout, device memory) --- */
 /* stream: a hipStream_t (NULL = the context's own stream) */
@@ -167,17 +171,72 @@ int hb_cg_multi_d(hb_context *ctx, int op, int nrhs, const double *kappa, const 
 /* BQCD layout <-> SoA conversions of one spinor (host <-> device) */
 int hb_spinor_to_device(hb_context *ctx, double *dst_d, const double *src_host);
 int hb_spinor_to_host(hb_context *ctx, double *dst_host, const double *src_d);
-/* dsf / autograd helpers on device spinors; p_d is SoA momenta */
+/* forces and link update on device fields; p_d is SoA momenta [4][2][8][volh] */
 int hb_dsf_d(hb_context *ctx, double *p_d, double step, double kappa, double csw_kappa,
              const double *a_d, const double *b_d);
+int hb_dsd_d(hb_context *ctx, double *p_d, double step, double csw_kappa);
+int hb_dsg_d(hb_context *ctx, double *p_d, double step, double beta);
+int hb_hmc_u_d(hb_context *ctx, const double *p_d, double step, double csw_kappa);
 /* synthetic fields on the device (seeded, deterministic on a given GPU) */
 int hb_gauge_random(hb_context *ctx, unsigned long long seed, double scale);
 int hb_gauge_smear(hb_context *ctx, int sweeps, double alpha);
-int hb_spinor_random_d(hb_context *ctx, double *s_d, unsigned long long seed);
 
 /* timing of the last solve: kernel time (ms) of the hopping-matrix launches
  * and their count, measured with HIP events on the context stream */
 int hb_last_timing(hb_context *ctx, double *solve_ms, int *n_matvec);
+
+/* ---- HMC trajectory: replaces hmc() (hmc.F90:28-180) with the integrator
+ *      of hmc_leap_frog.F90:28-79 / hmc_integrator.F90:28-103 ------------- */
+enum { HB_INT_LEAPFROG = 0,   /* hmc_leap_frog.F90: multi-time-scale leapfrog */
+       HB_INT_OMELYAN = 1 };  /* second-order minimum-norm (Omelyan) on both scales */
+
+typedef struct {
+  double beta, kappa, csw, rho;   /* hmc_para; csw = 0: no clover term */
+  double tau;                     /* step of the outer (ir) scale: trajectory length / ntau */
+  int ntau, m_scale;              /* outer steps; inner (uv) steps per outer step */
+  char model;                     /* 'A', 'B' or 'C' (hmc_integrator.F90) */
+  int integrator;                 /* HB_INT_* */
+  double lambda;                  /* Omelyan parameter (0 = 0.1931833275037836) */
+  float rest;                     /* cg_para%rest (absolute |r|^2) */
+  int maxiter;                    /* cg_para%maxiter */
+  int precision;                  /* solver: HB_PREC_F64 (cg.F90), HB_PREC_F32, HB_PREC_F16 */
+  double delta;                   /* reliable-update threshold of the mixed solver */
+  int max_inner;
+  int mre_vectors;                /* chronological guesses (mre.F90); 0: the source */
+  int batch;                      /* model B: solve the two pseudofermions together
+                                     (multi-RHS solver, precision F32/F16) */
+} hb_hmc_params;
+
+typedef struct {
+  double exp_dh, h_old, h_new;
+  double sg_old, sg_new;          /* plaquette action without beta: 6V - sum Re Tr U_p / 3 */
+  double sp_old, sp_new, sd_old, sd_new;
+  double sf1_old, sf1_new, sf2_old, sf2_new;
+  double plaquette;               /* <Re Tr U_p>/3 after the trajectory (accepted or not) */
+  int accepted;
+  int cg_ncall, cg_niter_tot, cg_niter_max;  /* get_cg_stat (cg.F90:166) */
+  int n_force_ir, n_force_uv;     /* force evaluations per scale */
+  double seconds;                 /* wall time of the trajectory */
+} hb_hmc_out;
+
+/* One trajectory from the loaded links (physical; the clover term is rebuilt
+ * from them).  p: momenta, eta1/eta2: pseudofermion noise of hmc_init_phi
+ * (eta2 is read only for models B and C), in BQCD layouts (p with
+ * site_stride); they are the draws of hmc_init_p / hmc_init_phi, so a caller
+ * that draws them with BQCD's ranf reproduces BQCD's trajectory.
+ * accept_random < 0 forces acceptance; otherwise the trajectory is accepted
+ * when accept_random < exp(-Delta H), and on rejection the links (and clover
+ * term) of the start are restored. */
+int hb_hmc_trajectory(hb_context *ctx, const hb_hmc_params *prm, const double *p, int site_stride,
+                      const double *eta1, const double *eta2, double accept_random, hb_hmc_out *out);
+/* the same on device fields: p_d [4][2][8][volh], eta*_d [12][volh][2] */
+int hb_hmc_trajectory_d(hb_context *ctx, const hb_hmc_params *prm, const double *p_d,
+                        const double *eta1_d, const double *eta2_d, double accept_random,
+                        hb_hmc_out *out);
+/* n Gaussian numbers of variance var on the device (seeded, counter-based):
+ * momenta are var 1 (hmc_init_p), pseudofermion noise var 1/2 per real
+ * component (ran_gauss_volh(12, ., 1/2, .)) */
+int hb_gaussian_d(hb_context *ctx, unsigned long long seed, double var, double *x_d, long long n);
 
 #ifdef __cplusplus
 }

@@ -67,6 +67,7 @@ def lib():
         L.or_ranset.argtypes = [ctypes.c_int64, ctypes.c_int64]
         L.or_ranget.restype = ctypes.c_int64
         L.or_ranf.restype = ctypes.c_double
+        L.or_ran_gauss_volh.argtypes = [ctypes.POINTER(Geom), ctypes.c_int, _dp, ctypes.c_double, ctypes.c_int]
         L.or_gauge_exp_random.argtypes = [ctypes.POINTER(Geom), _dp, ctypes.c_int64, ctypes.c_double]
         L.or_spinor_gauss.argtypes = [ctypes.POINTER(Geom), _dp, ctypes.c_int64, ctypes.c_int]
         L.or_momenta_gauss.argtypes = [ctypes.POINTER(Geom), _dp, ctypes.c_int64]
@@ -119,6 +120,27 @@ class Lattice:
         s = self.spinor()
         lib().or_spinor_gauss(ctypes.byref(self.g), _p(s), seed, eo)
         return s
+
+    def ran_gauss_volh(self, m, var, eo=0):
+        """ran_gauss_volh (ran.F90:28-110) continuing the current ranf stream:
+        m complex Gaussians per site of variance var per real component"""
+        out = np.zeros((self.volh, m, 2))
+        lib().or_ran_gauss_volh(ctypes.byref(self.g), m, _p(out), var, eo)
+        return out
+
+    def hmc_draws(self, seed, hasenbusch=True):
+        """the random fields of one trajectory of hmc.F90 in its order of draws
+        after ranset(seed): momenta (hmc_init_p.F90: 8 ran_gauss_volh(4, ., 1)
+        for mu, eo), then the pseudofermion noise eta1 and, with Hasenbusch,
+        eta2 (hmc_init_phi.F90: ran_gauss_volh(12, ., 1/2, EVEN))"""
+        lib().or_ranset(seed, 0)
+        p = np.zeros((4, 2, self.volh, 8))
+        for mu in range(4):
+            for eo in range(2):
+                p[mu, eo] = self.ran_gauss_volh(4, 1.0, eo).reshape(self.volh, 8)
+        eta1 = self.ran_gauss_volh(12, 0.5).reshape(self.volh, 3, 4, 2)
+        eta2 = self.ran_gauss_volh(12, 0.5).reshape(self.volh, 3, 4, 2) if hasenbusch else None
+        return p, eta1, eta2
 
     def momenta_gauss(self, seed):
         p = self.gen()

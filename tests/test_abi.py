@@ -1,0 +1,59 @@
+"""C-ABI checks that need no GPU: the library loads (its HIP runtime
+dependency resolves without a device) and exports every entry point that
+include/bqcd_hip.h declares, and nothing on the host side falls back to the
+oracle when the library is absent."""
+import ctypes
+import os
+import re
+
+import pytest
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+REPO = os.path.dirname(HERE)
+HEADER = os.path.join(REPO, "include", "bqcd_hip.h")
+LIB = os.path.join(REPO, "bqcd-r399-quda_amd", "libbqcd_hip.so")
+
+
+def declared():
+    text = open(HEADER).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    names = re.findall(r"^\s*(?:const\s+)?\w+\s*\*?\s*(hb_\w+)\s*\(", text, flags=re.M)
+    return sorted(set(names))
+
+
+def test_header_parses():
+    names = declared()
+    for must in ("hb_init", "hb_cg", "hb_cg_mixed", "hb_cg_multi", "hb_dslash", "hb_apply",
+                 "hb_dsf", "hb_dsd", "hb_dsg", "hb_hmc_u", "hb_clover_mult", "hb_build_clover"):
+        assert must in names
+    assert len(names) >= 39
+
+
+@pytest.mark.skipif(not os.path.exists(LIB), reason="libbqcd_hip.so not built")
+def test_library_exports_every_declared_symbol():
+    lib = ctypes.CDLL(LIB)
+    missing = [n for n in declared() if not hasattr(lib, n)]
+    assert not missing, missing
+
+
+@pytest.mark.skipif(not os.path.exists(LIB), reason="libbqcd_hip.so not built")
+def test_error_path_without_device():
+    """hb_init on a host with no usable device (or a bad argument) returns an
+    error code and a message instead of crashing."""
+    lib = ctypes.CDLL(LIB)
+    lib.hb_last_error.restype = ctypes.c_char_p
+    L = (ctypes.c_int * 4)(4, 4, 4, 3)          # odd extent: rejected before any HIP call
+    bc = (ctypes.c_int * 4)(1, 1, 1, -1)
+    ctx = ctypes.c_void_p()
+    rc = lib.hb_init(L, bc, 0, ctypes.byref(ctx))
+    assert rc == -1
+    assert lib.hb_last_error()
+
+
+def test_harness_fails_loudly_without_library(monkeypatch):
+    import bqcd_hip
+
+    monkeypatch.setattr(bqcd_hip, "LIB_PATH", os.path.join(HERE, "no_such_lib.so"))
+    monkeypatch.setattr(bqcd_hip, "_lib", None, raising=False)
+    with pytest.raises(bqcd_hip.HipError):
+        bqcd_hip.lib()

@@ -1,0 +1,129 @@
+"""The HMC trajectory on the GPU (hb_hmc_trajectory) against the reference's
+own hmc() on identical start links and identical random draws (the momenta
+and pseudofermion noise that hmc() draws from ranf after ranset(seed), made
+here by the oracle's ranf, which reproduces ranf_test.reference exactly):
+
+ - traj_quick: the reference's work/input.quick-test (cold start, model C,
+   MRE with 7 vectors, solver bound 1e-5); its golden value is the one the
+   reference itself printed (work/out.quick-test.reference:44);
+ - traj_b: model B on a rough start, tight solver, no MRE;
+ - traj_a: model A with MRE 3.
+Bars: exp(-Delta H) to 1e-9 relative, final plaquette action and links to
+1e-10, identical CG call and iteration counts.
+
+Then properties no reference pins: Omelyan and leapfrog Delta H fall as
+step^2; batched (multi-RHS) and sequential mixed-precision model-B
+trajectories agree; rejection restores the start links."""
+import ast
+
+import numpy as np
+import pytest
+
+import oracle_lib as ol
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def hip():
+    import bqcd_hip
+    return bqcd_hip
+
+
+def start_links(lat, f):
+    start = str(f["start"])
+    if start == "cold":
+        u = lat.gauge()
+        for k in range(3):
+            u[:, :, :, k, k, 0] = 1.0
+        return u
+    seed, scale = ast.literal_eval(start)  # the (seed, scale) tuple of make_golden.py
+    return lat.gauge_exp_random(int(seed), float(scale))
+
+
+def params(hip, f, **kw):
+    d = dict(beta=float(f["beta"]), kappa=float(f["kappa"]), csw=float(f["csw"]), rho=float(f["rho"]),
+             tau=float(f["tau"]), ntau=int(f["ntau"]), m_scale=int(f["m_scale"]), model=str(f["model"]),
+             rest=float(f["rest"]), maxiter=int(f["maxiter"]), mre_vectors=int(f["mre"]))
+    d.update(kw)
+    return hip.hmc_params(**d)
+
+
+@pytest.mark.parametrize("case", ["traj_quick", "traj_b", "traj_a"])
+def test_trajectory_vs_reference(hip, golden, case):
+    f = golden(case)
+    L = tuple(int(x) for x in f["L"])
+    lat = ol.Lattice(L)
+    u0 = start_links(lat, f)
+    p, eta1, eta2 = lat.hmc_draws(int(f["seed"]), hasenbusch=str(f["model"]) != "A")
+    g = hip.Lattice(L)
+    g.load_gauge(u0)
+    print(case, flush=True)
+    out = g.hmc_trajectory(params(hip, f), p, eta1, eta2)
+    print(out, flush=True)
+    assert abs(out["exp_dh"] - float(f["info_exp_dh"])) < 1e-9 * float(f["info_exp_dh"])
+    assert abs(out["sg_new"] - float(f["info_sg_new"])) < 1e-10 * float(f["info_sg_new"])
+    assert out["cg_ncall"] == int(f["info_cg_ncall"])
+    assert out["cg_niter_tot"] == int(f["info_cg_niter_tot"])
+    assert out["cg_niter_max"] == int(f["info_cg_niter_max"])
+    u = g.save_gauge()
+    assert np.max(np.abs(u - f["u_new"])) < 1e-10
+    g.close()
+
+
+def dh_at(hip, g, u0, prm, draws):
+    g.load_gauge(u0)
+    out = g.hmc_trajectory(prm, *draws)
+    return out["h_new"] - out["h_old"]
+
+
+@pytest.mark.parametrize("integrator", ["leapfrog", "omelyan"])
+def test_delta_h_scales_as_step_squared(hip, integrator):
+    L = (4, 4, 4, 4)
+    lat = ol.Lattice(L)
+    u0 = lat.gauge_exp_random(5, 0.2)
+    draws = lat.hmc_draws(77)
+    g = hip.Lattice(L)
+    dh = []
+    for n in (4, 8):
+        prm = hip.hmc_params(beta=5.29, kappa=0.13, csw=1.9, rho=0.1, tau=0.2 / n, ntau=n, m_scale=2,
+                             model="B", integrator=integrator, rest=1e-20, maxiter=5000)
+        dh.append(dh_at(hip, g, u0, prm, draws))
+    print(integrator, dh, dh[0] / dh[1], flush=True)
+    assert 3.0 < dh[0] / dh[1] < 5.5
+    g.close()
+
+
+def test_batched_model_b_matches_sequential(hip):
+    L = (4, 4, 4, 6)
+    lat = ol.Lattice(L)
+    u0 = lat.gauge_exp_random(6, 0.2)
+    draws = lat.hmc_draws(78)
+    g = hip.Lattice(L)
+    res = {}
+    for batch in (False, True):
+        prm = hip.hmc_params(beta=5.29, kappa=0.13, csw=1.9, rho=0.1, tau=0.05, ntau=4, m_scale=2,
+                             model="B", integrator="omelyan", rest=1e-20, maxiter=5000,
+                             precision="f32", batch=batch)
+        g.load_gauge(u0)
+        res[batch] = (g.hmc_trajectory(prm, *draws), g.save_gauge())
+    a, b = res[False], res[True]
+    print(a[0], b[0], flush=True)
+    assert abs(a[0]["h_new"] - a[0]["h_old"] - (b[0]["h_new"] - b[0]["h_old"])) < 1e-7
+    assert np.max(np.abs(a[1] - b[1])) < 1e-9
+    g.close()
+
+
+def test_rejection_restores_links(hip):
+    L = (4, 4, 4, 4)
+    lat = ol.Lattice(L)
+    u0 = lat.gauge_exp_random(8, 0.2)
+    draws = lat.hmc_draws(79)
+    g = hip.Lattice(L)
+    g.load_gauge(u0)
+    prm = hip.hmc_params(beta=5.29, kappa=0.13, csw=1.9, rho=0.1, tau=0.05, ntau=2, model="C",
+                         rest=1e-22, maxiter=5000)
+    out = g.hmc_trajectory(prm, *draws, accept_random=2.0e300)
+    assert out["accepted"] == 0
+    assert np.array_equal(g.save_gauge(), u0)
+    g.close()

@@ -135,3 +135,18 @@ def test_oracle_sampled_8888(golden):
     assert rel(p[:, :, sites], f["dsf_sample"]) < TOL
     p = lat.dsd(b, float(f["step"]), csw * kappa, u)
     assert rel(p[:, :, sites], f["dsd_sample"]) < TOL
+
+
+def test_golden_trajectory_is_the_reference_quick_test(golden):
+    """The golden trajectory traj_quick (the reference's hmc() run by
+    oracle/ref/traj_driver.F90 with the parameters of work/input.quick-test)
+    reproduces the reference's own printed result for that input,
+    work/out.quick-test.reference:44: plaquette energy 0.2460441746,
+    exp(-Delta H) 0.4559411236, 43 CG calls, 133 iterations, at most 8."""
+    f = golden("traj_quick")
+    vol = 4 ** 4
+    assert round(float(f["info_sg_new"]) / (6 * vol), 10) == 0.2460441746
+    assert round(float(f["info_exp_dh"]), 10) == 0.4559411236
+    assert int(f["info_cg_ncall"]) == 43
+    assert int(f["info_cg_niter_tot"]) == 133
+    assert int(f["info_cg_niter_max"]) == 8
