@@ -48,6 +48,10 @@ def parse():
     ap.add_argument("--dslash-reps", type=int, default=50)
     ap.add_argument("--cpu-iters", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-sweep", action="store_true",
+                    help="skip the precision comparison at the configuration's reliable-update "
+                         "settings (FP32 and 16-bit inner, update every 100 iterations or on a "
+                         "1e-2 residual drop)")
     ap.add_argument("--no-batch", action="store_true",
                     help="skip the multi-RHS workload (24^3x48, 12 RHS over 3 kappa)")
     ap.add_argument("--batch-lattice", default="24,24,24,48")
@@ -339,6 +343,34 @@ def main():
     if true_rel > 1e-20 * 1.01:
         log("WARNING: true residual above 1e-20 |b|^2")
 
+    # precision comparison (FP32 inner, then 16-bit storage) at the reliable-update
+    # settings the precision configuration prescribes: update every 100 inner
+    # iterations or when the residual drops by 1e-2; each against the FP64 solve
+    sweep = None
+    if not args.no_sweep and fp64:
+        sweep = {}
+        for name, prec, delta, mi in (("f32_delta1e-2_every100", "f32", 1e-2, 100),
+                                      ("f16_delta1e-2_every100", "f16", 1e-2, 100),
+                                      ("f32_delta1e-1_every1000", "f32", 0.1, 1000),
+                                      ("f16_delta1e-1_every1000", "f16", 0.1, 1000)):
+            x.zero_()
+            torch.cuda.synchronize()
+            q0, q1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            q0.record(stream)
+            it_s, nrel_s, tr_s = g.cg_mixed_d("mtdagmt", x, b, kappa, rest=rest, maxiter=40000,
+                                              precision=prec, delta=delta, max_inner=mi)
+            q1.record(stream)
+            torch.cuda.synchronize()
+            ms_s = q0.elapsed_time(q1)
+            sweep[name] = {
+                "s": ms_s / 1e3, "iterations": it_s, "reliable_updates": nrel_s,
+                "true_residual_rel": tr_s / bnorm2,
+                "solution_rel_error_vs_fp64": float(torch.linalg.vector_norm(x - fp64["x"])
+                                                    / torch.linalg.vector_norm(fp64["x"])),
+                "speedup_vs_fp64": fp64["ms"] / ms_s,
+            }
+            log("precision", name, json.dumps(sweep[name]))
+
     cpu = None
     if not args.no_cpu:
         try:
@@ -392,6 +424,7 @@ def main():
             "bytes_per_launch": ds_bytes,
         },
         "cpu_baseline": cpu,
+        "precision_comparison": sweep,
         "batched": batch,
     }
     print(json.dumps(line), flush=True)

@@ -78,6 +78,14 @@ _lib = None
 def lib():
     global _lib
     if _lib is None:
+        # One HIP runtime per process: PyTorch-ROCm carries its own copy of
+        # libamdhip64.so.7.  Loaded first, it is the one this library binds to
+        # (same soname); loaded second, it would start a second runtime that
+        # sees no GPU.  So torch, when installed, is loaded before the library.
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         if not os.path.exists(LIB_PATH):
             raise HipError("libbqcd_hip.so not built (run __graft_entry__.build()): %s" % LIB_PATH)
         L = ctypes.CDLL(LIB_PATH)

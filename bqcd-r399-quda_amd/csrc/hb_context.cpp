@@ -92,7 +92,9 @@ int hb_init(const int L[4], const int bc[4], int device, hb_context **out) {
   c->g.nxh = L[0] / 2;
   c->g.vh = (int)(vol / 2);
   c->device = device;
-  hipError_t e = hipStreamCreateWithFlags(&c->own_stream, hipStreamNonBlocking);
+  // a blocking stream: ordered with the null stream, so a caller that works on
+  // the null stream (and passes no stream of its own) needs no extra waits
+  hipError_t e = hipStreamCreate(&c->own_stream);
   if (e != hipSuccess) {
     delete c;
     set_error("hipStreamCreate: %s", hipGetErrorString(e));

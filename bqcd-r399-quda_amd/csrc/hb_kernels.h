@@ -44,6 +44,7 @@ struct SF64 {
   using V = double2;
   using ST = double2;
   static constexpr bool kNorm = false;
+  static constexpr bool kRecon = false;  // links stored with all three rows
   static constexpr int kBytes = 16;
   __device__ __forceinline__ static V ld(const ST *p, size_t k) { return p[k]; }
   __device__ __forceinline__ static void st(ST *p, size_t k, V v) { p[k] = v; }
@@ -56,6 +57,7 @@ struct SF32 {
   using V = float2;
   using ST = float2;
   static constexpr bool kNorm = false;
+  static constexpr bool kRecon = true;   // links stored as two rows, third rebuilt
   static constexpr int kBytes = 8;
   __device__ __forceinline__ static V ld(const ST *p, size_t k) { return p[k]; }
   __device__ __forceinline__ static void st(ST *p, size_t k, V v) { p[k] = v; }
@@ -73,6 +75,7 @@ struct SF16 {
   using V = float2;
   using ST = short2;
   static constexpr bool kNorm = true;
+  static constexpr bool kRecon = true;
   static constexpr int kBytes = 4;
   static constexpr float kScale = 32767.0f;
   static constexpr float kInv = 1.0f / 32767.0f;
@@ -161,16 +164,33 @@ __device__ __forceinline__ void store_spinor(const SpinorPtr<S> &f, int i, int v
 }
 
 // ----------------------------------------------------------------------------
-// links: U[4][2][9][vh], element 3*row + col; one buffer per (mu, eo) slab
+// links: U[4][2][9][vh], element 3*row + col; one buffer per (mu, eo) slab.
+// Reduced-precision copies keep rows 0 and 1 only (U[4][2][6][vh]): for
+// U in SU(3) the third row is conj(row0 x row1), rebuilt after the load, so a
+// link costs 6 instead of 9 complex numbers of traffic.
 // ----------------------------------------------------------------------------
+template <class S> constexpr int link_planes() { return S::kRecon ? 6 : 9; }
+
 template <class S, int POL = kNT>
 __device__ __forceinline__ void load_link(typename S::V (&u)[9], const typename S::ST *U, int mu,
                                           int eo, int i, int vh) {
+  constexpr int NP = link_planes<S>();
   const uint32_t plane = (uint32_t)vh * S::kBytes;
-  rsrc_t r = make_rsrc(U + (size_t)((mu * 2 + eo) * 9) * vh, plane * 9u);
+  rsrc_t r = make_rsrc(U + (size_t)((mu * 2 + eo) * NP) * vh, plane * (uint32_t)NP);
   const uint32_t vo = (uint32_t)i * S::kBytes;
 #pragma unroll
-  for (int k = 0; k < 9; k++) u[k] = S::template bld<POL>(r, vo, plane * k);
+  for (int k = 0; k < NP; k++) u[k] = S::template bld<POL>(r, vo, plane * k);
+  if constexpr (S::kRecon) {
+    using V = typename S::V;
+    auto cross = [](V a0, V b1, V a1, V b0) {  // conj(a0 b1 - a1 b0)
+      V t = cmul(a0, b1);
+      V s = cmul(a1, b0);
+      return cmk<V>(t.x - s.x, -(t.y - s.y));
+    };
+    u[6] = cross(u[1], u[5], u[2], u[4]);
+    u[7] = cross(u[2], u[3], u[0], u[5]);
+    u[8] = cross(u[0], u[4], u[1], u[3]);
+  }
 }
 
 // w = U h  (h: 3 colours x 2 spins, index 2*c + s)

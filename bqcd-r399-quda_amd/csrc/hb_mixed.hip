@@ -35,6 +35,17 @@ __global__ void k_cvt_field(typename S::ST *__restrict__ dst, const double2 *__r
   }
 }
 
+// links in reduced precision, rows 0 and 1 only (U[4][2][6][vh], see load_link)
+template <class S>
+__global__ void k_cvt_gauge_rows(typename S::ST *__restrict__ dst, const double2 *__restrict__ src, size_t vh) {
+  const size_t n = 48 * vh;
+  for (size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += (size_t)gridDim.x * blockDim.x) {
+    const size_t slab = k / (6 * vh), rem = k - slab * 6 * vh;
+    const double2 v = src[slab * 9 * vh + rem];
+    S::st(dst, k, cmk<typename S::V>((typename S::A)v.x, (typename S::A)v.y));
+  }
+}
+
 // clover in 16-bit storage: per site and block, scale by the largest magnitude
 __global__ void k_cvt_clover16(short2 *__restrict__ dst, float *__restrict__ nrm,
                                const double2 *__restrict__ src, int vh) {
@@ -240,8 +251,8 @@ int ensure_lowprec(Context *c, int prec) {
     return HB_OK;
   const int grid = nblocks((int)std::min<size_t>(72 * vh, (size_t)1 << 22));
   if (prec == 32) {
-    if (!c->gauge32) HB_HIP(hipMalloc(&c->gauge32, sizeof(float2) * 72 * vh));
-    k_cvt_field<SF32><<<grid, kBlock, 0, c->stream>>>(c->gauge32, c->gauge, 72 * vh);
+    if (!c->gauge32) HB_HIP(hipMalloc(&c->gauge32, sizeof(float2) * 48 * vh));
+    k_cvt_gauge_rows<SF32><<<grid, kBlock, 0, c->stream>>>(c->gauge32, c->gauge, vh);
     HB_HIP(hipGetLastError());
     if (clov) {
       for (int k = 0; k < 2; k++) {
@@ -251,8 +262,8 @@ int ensure_lowprec(Context *c, int prec) {
       }
     }
   } else {
-    if (!c->gauge16) HB_HIP(hipMalloc(&c->gauge16, sizeof(short2) * 72 * vh));
-    k_cvt_field<SF16><<<grid, kBlock, 0, c->stream>>>(c->gauge16, c->gauge, 72 * vh);
+    if (!c->gauge16) HB_HIP(hipMalloc(&c->gauge16, sizeof(short2) * 48 * vh));
+    k_cvt_gauge_rows<SF16><<<grid, kBlock, 0, c->stream>>>(c->gauge16, c->gauge, vh);
     HB_HIP(hipGetLastError());
     if (clov) {
       for (int k = 0; k < 2; k++) {

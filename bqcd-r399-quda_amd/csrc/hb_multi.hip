@@ -367,11 +367,15 @@ __global__ void k_cvt_clover_lp(typename S::ST *__restrict__ dst, float *__restr
   }
 }
 
+// links in reduced precision, rows 0 and 1 only (U[4][2][6][vh], see load_link)
 template <class S>
-__global__ void k_cvt_gauge_lp(typename S::ST *__restrict__ dst, const double2 *__restrict__ src, size_t n) {
-  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  for (size_t k = i; k < n; k += (size_t)gridDim.x * blockDim.x)
-    S::st(dst, k, make_float2((float)src[k].x, (float)src[k].y));
+__global__ void k_cvt_gauge_lp(typename S::ST *__restrict__ dst, const double2 *__restrict__ src, size_t vh) {
+  const size_t n = 48 * vh;
+  for (size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += (size_t)gridDim.x * blockDim.x) {
+    const size_t slab = k / (6 * vh), rem = k - slab * 6 * vh;
+    const double2 v = src[slab * 9 * vh + rem];
+    S::st(dst, k, make_float2((float)v.x, (float)v.y));
+  }
 }
 
 // kernel building the clover term into given arrays (hb_gauge.hip)
@@ -480,9 +484,10 @@ template <class S, class G>
 int multi_fields(Context *c, const MultiPlan &plan, MultiFields<S, G> &f) {
   const size_t vh = c->g.vh;
   int rc = 0;
-  typename G::ST *U = static_cast<typename G::ST *>(scratch(c, SLOT_MULTI + 20, st_bytes<G>() * 72 * vh, &rc));
+  static_assert(G::kRecon, "reduced-precision links are stored as two rows");
+  typename G::ST *U = static_cast<typename G::ST *>(scratch(c, SLOT_MULTI + 20, st_bytes<G>() * 48 * vh, &rc));
   HB_TRY(rc);
-  k_cvt_gauge_lp<G><<<nblocks((int)std::min<size_t>(72 * vh, (size_t)1 << 22)), kBlock, 0, c->stream>>>(U, c->gauge, 72 * vh);
+  k_cvt_gauge_lp<G><<<nblocks((int)std::min<size_t>(48 * vh, (size_t)1 << 22)), kBlock, 0, c->stream>>>(U, c->gauge, vh);
   HB_HIP(hipGetLastError());
   f.U = U;
   if (!plan.clover) return HB_OK;

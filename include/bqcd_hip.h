@@ -6,8 +6,9 @@
  *
  * Plain C types only.  Every entry point returns 0 on success or a negative
  * HB_E_* code; hb_last_error() gives a message.  A Fortran iso_c_binding
- * module (bqcd-r399-quda_amd/fortran/bqcd_hip_solver.F90) maps these onto
- * BQCD's cg() / quda_solver() interfaces.
+ * module (bqcd-r399-quda_amd/fortran/bqcd_hip.F90) declares every entry
+ * point; bqcd-r399-quda_amd/fortran/hip_solver.F90 maps them onto BQCD's
+ * cg() / quda_solver(), dsf, dsd, dsg and hmc_u interfaces.
  *
  * Host-side field layouts are BQCD's own (Fortran column-major):
  *   spinor   complex(4 spin, 3 colour, volh)                SPINCOL_FIELD
@@ -155,7 +156,9 @@ int hb_gauge_backup(hb_context *ctx, int slot);
 int hb_gauge_restore(hb_context *ctx, int slot);
 
 /* ---- device-pointer entry points (library SoA layout, device memory) --- */
-/* stream: a hipStream_t (NULL = the context's own stream) */
+/* stream: a hipStream_t, or NULL for the context's own stream, a blocking
+ * stream (ordered with the null stream).  All later work of the context is
+ * issued on it. */
 int hb_set_stream(hb_context *ctx, void *stream);
 int hb_dslash_d(hb_context *ctx, int out_parity, int dagger, double *out_d, const double *in_d);
 int hb_apply_d(hb_context *ctx, int op, double kappa, double rho, double *out_d,

@@ -85,8 +85,8 @@ def test_delta_h_scales_as_step_squared(hip, integrator):
     draws = lat.hmc_draws(77)
     g = hip.Lattice(L)
     dh = []
-    for n in (4, 8):
-        prm = hip.hmc_params(beta=5.29, kappa=0.13, csw=1.9, rho=0.1, tau=0.2 / n, ntau=n, m_scale=2,
+    for n in (16, 32):
+        prm = hip.hmc_params(beta=5.29, kappa=0.13, csw=1.9, rho=0.1, tau=0.2 / n, ntau=n, m_scale=1,
                              model="B", integrator=integrator, rest=1e-20, maxiter=5000)
         dh.append(dh_at(hip, g, u0, prm, draws))
     print(integrator, dh, dh[0] / dh[1], flush=True)
