@@ -37,6 +37,8 @@ int clover_mult_f64(Context *c, int which, int eo, double2 *out, const double2 *
 // coalesced planes (component k of site i at k*vh + i).
 #define HB_SITE_LOOP for (int k = 0; k < 12; k++)
 #define HB_AT(ptr, k) (ptr)[(size_t)(k) * vh + i]
+// streaming (non-temporal) store of one complex number, see store_spinor
+#define HB_ST(ptr, k, v) SF64::st_nt((ptr), (size_t)(k) * vh + i, (v))
 
 // r = b - r, p = r, rtr_old = |r|^2  (cg.F90:84-89)
 __global__ __launch_bounds__(kBlock) void k_cg_start(double2 *__restrict__ r, double2 *__restrict__ p,
@@ -73,8 +75,8 @@ __global__ __launch_bounds__(kBlock) void k_cg_update(double2 *__restrict__ x, d
       double2 pp = HB_AT(p, k), qq = HB_AT(q, k), xx = HB_AT(x, k), rr = HB_AT(r, k);
       xx.x += alpha * pp.x; xx.y += alpha * pp.y;
       rr.x -= alpha * qq.x; rr.y -= alpha * qq.y;
-      HB_AT(x, k) = xx;
-      HB_AT(r, k) = rr;
+      HB_ST(x, k, xx);
+      HB_ST(r, k, rr);
       s += rr.x * rr.x + rr.y * rr.y;
     }
   }
@@ -92,7 +94,7 @@ __global__ __launch_bounds__(kBlock) void k_cg_pupdate(double2 *__restrict__ p, 
 #pragma unroll
   HB_SITE_LOOP {
     double2 pp = HB_AT(p, k), rr = HB_AT(r, k);
-    HB_AT(p, k) = make_double2(rr.x + beta * pp.x, rr.y + beta * pp.y);
+    HB_ST(p, k, make_double2(rr.x + beta * pp.x, rr.y + beta * pp.y));
   }
 }
 

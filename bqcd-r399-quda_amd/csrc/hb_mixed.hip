@@ -77,8 +77,8 @@ __global__ void k_cvt_spinor(SpinorPtr<S> dst, const double2 *__restrict__ src, 
 
 // ---- inner iteration kernels -----------------------------------------------------
 // alpha = rtr / pAp; x_lo += alpha p; r -= alpha q; rtr_new = |r|^2; decide
-template <class S>
-__global__ __launch_bounds__(kBlock) void k_mix_update(float2 *__restrict__ xlo, SpinorPtr<S> r,
+template <class S, class R>
+__global__ __launch_bounds__(kBlock) void k_mix_update(float2 *__restrict__ xlo, SpinorPtr<R> r,
                                                        SpinorPtr<S> p, SpinorPtr<S> q, int vh,
                                                        const MixState *st, double *partials) {
   using V = typename S::V;
@@ -90,7 +90,7 @@ __global__ __launch_bounds__(kBlock) void k_mix_update(float2 *__restrict__ xlo,
     V pv[12], qv[12], rv[12];
     load_spinor<S>(pv, p, i, vh);
     load_spinor<S>(qv, q, i, vh);
-    load_spinor<S>(rv, r, i, vh);
+    load_spinor<R>(rv, r, i, vh);
 #pragma unroll
     for (int k = 0; k < 12; k++) {
       float2 xv = xlo[(size_t)k * vh + i];
@@ -101,7 +101,7 @@ __global__ __launch_bounds__(kBlock) void k_mix_update(float2 *__restrict__ xlo,
       rv[k].y -= alpha * qv[k].y;
       s += (double)rv[k].x * rv[k].x + (double)rv[k].y * rv[k].y;
     }
-    store_spinor<S>(r, i, vh, rv);
+    store_spinor<R>(r, i, vh, rv);
   }
   double vals[1] = {s};
   block_partials<1>(vals, partials);
@@ -128,8 +128,8 @@ struct FinMixUpdate {
 };
 
 // p = r + beta p
-template <class S>
-__global__ __launch_bounds__(kBlock) void k_mix_pupdate(SpinorPtr<S> p, SpinorPtr<S> r, int vh,
+template <class S, class R>
+__global__ __launch_bounds__(kBlock) void k_mix_pupdate(SpinorPtr<S> p, SpinorPtr<R> r, int vh,
                                                         const MixState *st) {
   using V = typename S::V;
   if (st->flag) return;
@@ -138,7 +138,7 @@ __global__ __launch_bounds__(kBlock) void k_mix_pupdate(SpinorPtr<S> p, SpinorPt
   if (i >= vh) return;
   V pv[12], rv[12];
   load_spinor<S>(pv, p, i, vh);
-  load_spinor<S>(rv, r, i, vh);
+  load_spinor<R>(rv, r, i, vh);
 #pragma unroll
   for (int k = 0; k < 12; k++) {
     pv[k].x = rv[k].x + beta * pv[k].x;
@@ -160,10 +160,10 @@ __global__ void k_mix_accum(double2 *__restrict__ x, float2 *__restrict__ xlo, s
 }
 
 // r = b - Ax (in place in ax), |r|^2; then r_lo = r and the scalar bookkeeping
-template <class S>
+template <class R>
 __global__ __launch_bounds__(kBlock) void k_mix_true_residual(double2 *__restrict__ ax,
                                                               const double2 *__restrict__ b,
-                                                              SpinorPtr<S> rlo, int vh,
+                                                              SpinorPtr<R> rlo, int vh,
                                                               double *partials) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   double s = 0.0;
@@ -176,7 +176,7 @@ __global__ __launch_bounds__(kBlock) void k_mix_true_residual(double2 *__restric
       ax[(size_t)k * vh + i] = v[k];
       s += v[k].x * v[k].x + v[k].y * v[k].y;
     }
-    store_spinor<S>(rlo, i, vh, v);
+    store_spinor<R>(rlo, i, vh, v);
   }
   double vals[1] = {s};
   block_partials<1>(vals, partials);
@@ -206,8 +206,8 @@ struct FinMixTrue {
 };
 
 // p = r_lo + beta p after a reliable update (beta = 0 at the start: p = r)
-template <class S>
-__global__ __launch_bounds__(kBlock) void k_mix_prestart(SpinorPtr<S> p, SpinorPtr<S> r, int vh,
+template <class S, class R>
+__global__ __launch_bounds__(kBlock) void k_mix_prestart(SpinorPtr<S> p, SpinorPtr<R> r, int vh,
                                                          const MixState *st) {
   using V = typename S::V;
   if (st->flag) return;
@@ -215,7 +215,7 @@ __global__ __launch_bounds__(kBlock) void k_mix_prestart(SpinorPtr<S> p, SpinorP
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= vh) return;
   V pv[12], rv[12];
-  load_spinor<S>(rv, r, i, vh);
+  load_spinor<R>(rv, r, i, vh);
   if (beta != 0.f) {
     load_spinor<S>(pv, p, i, vh);
   } else {
@@ -329,7 +329,7 @@ template <class S> struct SpinorAlloc {
   }
 };
 
-template <class S, class G>
+template <class S, class G, class R = S>
 int cg_mixed_impl(Context *c, int op, double kappa, double rho, double2 *x, const double2 *b,
                   float rest, int maxiter, int log, double delta, int max_inner, int *iters,
                   int *n_rel, double *true_rtr) {
@@ -343,7 +343,7 @@ int cg_mixed_impl(Context *c, int op, double kappa, double rho, double2 *x, cons
   float2 *xlo = static_cast<float2 *>(scratch(c, SLOT_MIX1, sizeof(float2) * 12 * (size_t)vh, &rc)); HB_TRY(rc);
   // low-precision fields (norm slots follow SLOT_LPN)
   SpinorPtr<S> p = SpinorAlloc<S>::get(c, SLOT_LP0, SLOT_LPN, &rc); HB_TRY(rc);
-  SpinorPtr<S> r = SpinorAlloc<S>::get(c, SLOT_LP1, SLOT_LPN1, &rc); HB_TRY(rc);
+  SpinorPtr<R> r = SpinorAlloc<R>::get(c, SLOT_LP1, SLOT_LPN1, &rc); HB_TRY(rc);
   SpinorPtr<S> q = SpinorAlloc<S>::get(c, SLOT_LP2, SLOT_LPN2, &rc); HB_TRY(rc);
   SpinorPtr<S> t = SpinorAlloc<S>::get(c, SLOT_LP3, SLOT_LPN3, &rc); HB_TRY(rc);
   SpinorPtr<S> y = SpinorAlloc<S>::get(c, SLOT_LP4, SLOT_LPN4, &rc); HB_TRY(rc);
@@ -373,10 +373,10 @@ int cg_mixed_impl(Context *c, int op, double kappa, double rho, double2 *x, cons
       HB_HIP(hipGetLastError());
     }
     HB_TRY(apply_f64(c, op, kappa, rho, ax, x, t64, y64, nullptr, nullptr));
-    k_mix_true_residual<S><<<nblocks(vh), kBlock, 0, c->stream>>>(ax, b, r, vh, c->red);
+    k_mix_true_residual<R><<<nblocks(vh), kBlock, 0, c->stream>>>(ax, b, r, vh, c->red);
     k_finalize<1><<<1, 256, 0, c->stream>>>(c->red, nblocks(vh), nullptr, FinMixTrue{st, first});
     HB_HIP(hipGetLastError());
-    k_mix_prestart<S><<<nblocks(vh), kBlock, 0, c->stream>>>(p, r, vh, st);
+    k_mix_prestart<S, R><<<nblocks(vh), kBlock, 0, c->stream>>>(p, r, vh, st);
     HB_HIP(hipGetLastError());
     nmv++;
     return HB_OK;
@@ -388,10 +388,10 @@ int cg_mixed_impl(Context *c, int op, double kappa, double rho, double2 *x, cons
     for (int k = 0; k < chunk; k++) {
       HB_TRY((mtil_lp<S, G>(c, false, kappa, r_op, y, p, t, nullptr, SpinorPtr<S>{}, skip)));
       HB_TRY((mtil_lp<S, G>(c, true, kappa, r_op, q, y, t, &st->pap, p, skip)));
-      k_mix_update<S><<<nblocks(vh), kBlock, 0, c->stream>>>(xlo, r, p, q, vh, st, c->red);
+      k_mix_update<S, R><<<nblocks(vh), kBlock, 0, c->stream>>>(xlo, r, p, q, vh, st, c->red);
       k_finalize<1><<<1, 256, 0, c->stream>>>(c->red, nblocks(vh), &st->flag, FinMixUpdate{st});
       HB_HIP(hipGetLastError());
-      k_mix_pupdate<S><<<nblocks(vh), kBlock, 0, c->stream>>>(p, r, vh, st);
+      k_mix_pupdate<S, R><<<nblocks(vh), kBlock, 0, c->stream>>>(p, r, vh, st);
       HB_HIP(hipGetLastError());
     }
     HB_HIP(hipMemcpyAsync(hs, st, sizeof(MixState), hipMemcpyDeviceToHost, c->stream));
