@@ -61,6 +61,15 @@ def parse():
     ap.add_argument("--only-dslash", type=int, default=0,
                     help="run only N FP64 hopping-matrix launches and a calibration copy of the "
                          "same byte count (for hardware-counter runs), then exit")
+    ap.add_argument("--hmc", type=int, default=0, metavar="NTRAJ",
+                    help="run NTRAJ HMC trajectories of the trajectory workload (two Hasenbusch "
+                         "pseudofermions, Omelyan, tau=1, 20 outer steps) with the mixed-precision "
+                         "batched solver and with the FP64 solver, print the report, then exit")
+    ap.add_argument("--hmc-lattice", default="48,48,48,96")
+    ap.add_argument("--hmc-kappa", type=float, default=0.1365)
+    ap.add_argument("--hmc-rho", type=float, default=0.1)
+    ap.add_argument("--hmc-steps", type=int, default=20)
+    ap.add_argument("--hmc-mscale", type=int, default=2)
     ap.add_argument("--scan", action="store_true",
                     help="print plaquette and FP64 iteration counts per smearing sweep, then exit")
     return ap.parse_args()
@@ -200,6 +209,71 @@ def batched(args, torch, dev, stream):
     return res
 
 
+def hmc(args, torch, dev, stream):
+    """HMC trajectories: Wilson gauge action beta 5.29, clover csw 1.9, two
+    Hasenbusch pseudofermions (model B: S_f1 with W = M~ + rho, S_f2 the
+    ratio), Omelyan integrator, trajectory length 1 in --hmc-steps outer
+    steps with --hmc-mscale gauge steps each, Gaussian momenta and noise from
+    the device generator, Metropolis accept/reject.  The same trajectories
+    (same start, same draws) run with the mixed-precision batched solver and
+    with the FP64 solver."""
+    import bqcd_hip
+
+    L = tuple(int(v) for v in args.hmc_lattice.split(","))
+    beta, kappa, csw, rho = 5.29, args.hmc_kappa, args.csw, args.hmc_rho
+    g = bqcd_hip.Lattice(L)
+    g.set_stream(stream)
+    vh = g.volh
+    g.gauge_random(args.seed + 11, 0.0)
+    g.gauge_smear(args.sweeps, args.alpha)
+    g.gauge_backup(1)
+    p = torch.empty((4, 2, 8, vh), dtype=torch.float64, device=dev)
+    eta1 = torch.empty((12, vh, 2), dtype=torch.float64, device=dev)
+    eta2 = torch.empty_like(eta1)
+    rest = 1e-20 * 12 * vh  # |r|/|b| ~ 1e-10 for the unit-variance sources
+    rng = __import__("random").Random(args.seed + 12)
+    runs = {}
+    for name, prec, batch in (("mixed16_batched", "f16", True), ("fp64", "f64", False)):
+        g.gauge_restore(1)
+        prm = bqcd_hip.hmc_params(beta=beta, kappa=kappa, csw=csw, rho=rho, tau=1.0 / args.hmc_steps,
+                                  ntau=args.hmc_steps, m_scale=args.hmc_mscale, model="B",
+                                  integrator="omelyan", rest=rest, maxiter=50000, precision=prec,
+                                  mre_vectors=4, batch=batch)
+        trajs = []
+        for k in range(args.hmc):
+            seed = args.seed + 100 * k
+            g.gaussian_d(p, seed + 1, 1.0)
+            g.gaussian_d(eta1, seed + 2, 0.5)
+            g.gaussian_d(eta2, seed + 3, 0.5)
+            u = rng.random() if name == "mixed16_batched" else trajs_acc[k]
+            out = g.hmc_trajectory_d(prm, p, eta1, eta2, accept_random=u)
+            out["accept_random"] = u
+            log("hmc %s trajectory %d: dH %.6f exp(-dH) %.6f plaquette %.6f accepted %d, %d CG calls, "
+                "%d iterations, %.1f s" % (name, k, out["h_new"] - out["h_old"], out["exp_dh"],
+                                           out["plaquette"], out["accepted"], out["cg_ncall"],
+                                           out["cg_niter_tot"], out["seconds"]))
+            trajs.append(out)
+        if name == "mixed16_batched":
+            trajs_acc = [t["accept_random"] for t in trajs]
+        runs[name] = {
+            "delta_h": [t["h_new"] - t["h_old"] for t in trajs],
+            "plaquette": [t["plaquette"] for t in trajs],
+            "acceptance": sum(t["accepted"] for t in trajs) / max(1, len(trajs)),
+            "s_per_trajectory": sum(t["seconds"] for t in trajs) / max(1, len(trajs)),
+            "cg_iterations_per_trajectory": sum(t["cg_niter_tot"] for t in trajs) / max(1, len(trajs)),
+            "force_evaluations_ir_uv": [trajs[0]["n_force_ir"], trajs[0]["n_force_uv"]] if trajs else None,
+        }
+    res = {"workload": "HMC trajectories, clover-Wilson Nf=2 with two Hasenbusch pseudofermions, %s"
+                       % "x".join(map(str, L)),
+           "lattice": list(L), "beta": beta, "kappa": kappa, "csw": csw, "rho": rho,
+           "integrator": "Omelyan (2MN), tau 1, %d outer steps, %d gauge steps each"
+                         % (args.hmc_steps, args.hmc_mscale),
+           "trajectories": args.hmc, "runs": runs,
+           "speedup_vs_fp64": runs["fp64"]["s_per_trajectory"] / runs["mixed16_batched"]["s_per_trajectory"]}
+    g.close()
+    return res
+
+
 def scan(g, args, torch, dev):
     """plaquette and FP64 CG iterations after each smearing sweep"""
     gen = torch.Generator(device=dev)
@@ -228,6 +302,14 @@ def main():
         return 2
     import torch
     import bqcd_hip
+
+    if args.hmc:
+        dev = torch.device("cuda:0")
+        torch.cuda.set_device(dev)
+        stream = torch.cuda.Stream(device=dev)
+        torch.cuda.set_stream(stream)
+        print(json.dumps({"hmc": hmc(args, torch, dev, stream)}), flush=True)
+        return 0
 
     L = tuple(int(x) for x in args.lattice.split(","))
     kappa, csw = args.kappa, args.csw

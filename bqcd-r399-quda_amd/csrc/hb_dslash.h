@@ -15,7 +15,7 @@ template <class S, class G = S> struct DslashArgs {
   SpinorPtr<S> in;
   const typename G::ST *U;   // links [4][2][9][vh]
   const typename G::ST *C;   // clover field for the epilogue (A^-1 for CLINV, A for MTIL), or null
-  const float *Cn;           // FP16 clover scales
+  const float *Cn;           // 16-bit clover: (scale, diagonal offset) per site and block
   SpinorPtr<S> x0;           // MTIL: the vector the clover term acts on
   SpinorPtr<S> dotv;         // MTDOT: Re<dotv, out>
   double mkappa2;            // -kappa^2

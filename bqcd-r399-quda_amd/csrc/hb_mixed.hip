@@ -46,22 +46,17 @@ __global__ void k_cvt_gauge_rows(typename S::ST *__restrict__ dst, const double2
   }
 }
 
-// clover in 16-bit storage: per site and block, scale by the largest magnitude
-__global__ void k_cvt_clover16(short2 *__restrict__ dst, float *__restrict__ nrm,
+// clover in 16-bit storage: per site and block, c 1 + m Q (see clover_block_scale)
+__global__ void k_cvt_clover16(short2 *__restrict__ dst, float2 *__restrict__ nrm,
                                const double2 *__restrict__ src, int vh) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   int eb = blockIdx.y;  // eo*2 + J
   if (i >= vh) return;
   const double2 *s = src + (size_t)eb * 18 * vh + i;
-  double m = 0.0;
-  for (int k = 0; k < 18; k++) m = fmax(m, fmax(fabs(s[(size_t)k * vh].x), fabs(s[(size_t)k * vh].y)));
-  float fm = (float)m;
-  float inv = fm > 0.f ? 1.0f / fm : 0.f;
-  nrm[(size_t)eb * vh + i] = fm;
-  for (int k = 0; k < 18; k++) {
-    double2 v = s[(size_t)k * vh];
-    SF16::st(dst, (size_t)eb * 18 * vh + (size_t)k * vh + i, make_float2((float)v.x * inv, (float)v.y * inv));
-  }
+  const float2 mc = clover_block_scale(s, vh);
+  nrm[(size_t)eb * vh + i] = mc;
+  for (int k = 0; k < 18; k++)
+    SF16::st(dst, (size_t)eb * 18 * vh + (size_t)k * vh + i, clover_block_word(s[(size_t)k * vh], k, mc));
 }
 
 // spinor FP64 -> S (thread per site: FP16 needs the site's norm)
@@ -268,8 +263,9 @@ int ensure_lowprec(Context *c, int prec) {
     if (clov) {
       for (int k = 0; k < 2; k++) {
         if (!c->clov16[k]) HB_HIP(hipMalloc(&c->clov16[k], sizeof(short2) * 72 * vh));
-        if (!c->clov16n[k]) HB_HIP(hipMalloc(&c->clov16n[k], sizeof(float) * 4 * vh));
-        k_cvt_clover16<<<dim3(nblocks((int)vh), 4), kBlock, 0, c->stream>>>(c->clov16[k], c->clov16n[k],
+        if (!c->clov16n[k]) HB_HIP(hipMalloc(&c->clov16n[k], sizeof(float2) * 4 * vh));
+        k_cvt_clover16<<<dim3(nblocks((int)vh), 4), kBlock, 0, c->stream>>>(c->clov16[k],
+                                                                          reinterpret_cast<float2 *>(c->clov16n[k]),
                                                                           c->clov[k], (int)vh);
         HB_HIP(hipGetLastError());
       }

@@ -38,7 +38,7 @@ template <class S, class G> struct MultiArgs {
   SpinorPtr<S> out, in, x0, dotv;   // block layout, NR RHS per site
   const typename G::ST *U;
   const typename G::ST *C;          // [ngroups][2][36][vh] (A^-1 for CLINV, A for MTIL)
-  const float *Cn;                  // [ngroups][2][2][vh]
+  const float *Cn;                  // [ngroups][2][2][vh] x (scale, diagonal offset)
   int ngroups;
   int grp[16];
   double mk2[16], rho[16];
@@ -59,9 +59,16 @@ __device__ __forceinline__ void load_clover_block_g(typename SG::V (&w)[18], con
 #pragma unroll
   for (int k = 0; k < 18; k++) w[k] = SG::template bld<kNT>(r, vo, plane * (base + k));
   if constexpr (SG::kNorm) {
-    float n = Cn[((size_t)gidx * 4 + eo * 2 + J) * vh + i];
+    const float2 mc = reinterpret_cast<const float2 *>(Cn)[((size_t)gidx * 4 + eo * 2 + J) * vh + i];
 #pragma unroll
-    for (int k = 0; k < 18; k++) { w[k].x *= n; w[k].y *= n; }
+    for (int k = 0; k < 18; k++) {
+      w[k].x *= mc.x;
+      w[k].y *= mc.x;
+      if (clover_diag_word(k)) {
+        w[k].x += mc.y;
+        w[k].y += mc.y;
+      }
+    }
   }
 }
 
@@ -354,16 +361,16 @@ __global__ void k_cvt_clover_lp(typename S::ST *__restrict__ dst, float *__restr
   const int eb = blockIdx.y;  // eo*2 + J
   if (i >= vh) return;
   const double2 *s = src + (size_t)eb * 18 * vh + i;
-  float inv = 1.f;
   if constexpr (S::kNorm) {
-    double m = 0.0;
-    for (int k = 0; k < 18; k++) m = fmax(m, fmax(fabs(s[(size_t)k * vh].x), fabs(s[(size_t)k * vh].y)));
-    nrm[(size_t)eb * vh + i] = (float)m;
-    inv = m > 0.0 ? (float)(1.0 / m) : 0.f;
-  }
-  for (int k = 0; k < 18; k++) {
-    double2 v = s[(size_t)k * vh];
-    S::st(dst, (size_t)eb * 18 * vh + (size_t)k * vh + i, make_float2((float)v.x * inv, (float)v.y * inv));
+    const float2 mc = clover_block_scale(s, vh);
+    reinterpret_cast<float2 *>(nrm)[(size_t)eb * vh + i] = mc;
+    for (int k = 0; k < 18; k++)
+      S::st(dst, (size_t)eb * 18 * vh + (size_t)k * vh + i, clover_block_word(s[(size_t)k * vh], k, mc));
+  } else {
+    for (int k = 0; k < 18; k++) {
+      double2 v = s[(size_t)k * vh];
+      S::st(dst, (size_t)eb * 18 * vh + (size_t)k * vh + i, make_float2((float)v.x, (float)v.y));
+    }
   }
 }
 
@@ -494,12 +501,12 @@ int multi_fields(Context *c, const MultiPlan &plan, MultiFields<S, G> &f) {
   for (int w = 0; w < 2; w++) {
     f.C[w] = static_cast<typename G::ST *>(scratch(c, SLOT_MULTI + 21 + w, st_bytes<G>() * 72 * vh * plan.ngroups, &rc));
     HB_TRY(rc);
-    f.Cn[w] = static_cast<float *>(scratch(c, SLOT_MULTI + 23 + w, sizeof(float) * 4 * vh * plan.ngroups, &rc));
+    f.Cn[w] = static_cast<float *>(scratch(c, SLOT_MULTI + 23 + w, sizeof(float2) * 4 * vh * plan.ngroups, &rc));
     HB_TRY(rc);
     for (int gi = 0; gi < plan.ngroups; gi++) {
       const double2 *src = w == 0 ? plan.clovA[gi] : plan.clovI[gi];
       k_cvt_clover_lp<G><<<dim3(nblocks((int)vh), 4), kBlock, 0, c->stream>>>(
-          f.C[w] + (size_t)gi * 72 * vh, f.Cn[w] + (size_t)gi * 4 * vh, src, (int)vh);
+          f.C[w] + (size_t)gi * 72 * vh, f.Cn[w] + (size_t)gi * 8 * vh, src, (int)vh);
       HB_HIP(hipGetLastError());
     }
   }
