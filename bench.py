@@ -70,9 +70,24 @@ def parse():
     ap.add_argument("--hmc-rho", type=float, default=0.1)
     ap.add_argument("--hmc-steps", type=int, default=20)
     ap.add_argument("--hmc-mscale", type=int, default=2)
+    ap.add_argument("--hmc-therm", type=int, default=3,
+                    help="force-accepted trajectories from the smeared start before the measured ones")
     ap.add_argument("--scan", action="store_true",
                     help="print plaquette and FP64 iteration counts per smearing sweep, then exit")
     return ap.parse_args()
+
+
+def measured_traffic(L):
+    """HBM bytes per launch of the FP64 hopping matrix from the committed
+    rocprofv3 FETCH_SIZE / WRITE_SIZE summary (calibrated against a device
+    copy), for the lattice that summary covers; None otherwise."""
+    path = os.path.join(REPO, "profiles", "dslash_fp64_%s.md" % "x".join(map(str, L)))
+    if not os.path.exists(path):
+        return None, None
+    for line in open(path):
+        if line.startswith("traffic_bytes_per_launch:"):
+            return float(line.split(":")[1]), os.path.relpath(path, REPO)
+    return None, None
 
 
 def log(*a):
@@ -226,12 +241,26 @@ def hmc(args, torch, dev, stream):
     vh = g.volh
     g.gauge_random(args.seed + 11, 0.0)
     g.gauge_smear(args.sweeps, args.alpha)
-    g.gauge_backup(1)
     p = torch.empty((4, 2, 8, vh), dtype=torch.float64, device=dev)
     eta1 = torch.empty((12, vh, 2), dtype=torch.float64, device=dev)
     eta2 = torch.empty_like(eta1)
     rest = 1e-20 * 12 * vh  # |r|/|b| ~ 1e-10 for the unit-variance sources
     rng = __import__("random").Random(args.seed + 12)
+    # the smeared start is far from equilibrium at beta 5.29: force-accepted
+    # trajectories (as the reference's hmc_accept_first) bring it closer
+    therm = bqcd_hip.hmc_params(beta=beta, kappa=kappa, csw=csw, rho=rho, tau=0.5 / args.hmc_steps,
+                                ntau=args.hmc_steps, m_scale=args.hmc_mscale, model="B",
+                                integrator="omelyan", rest=rest, maxiter=50000, precision="f16",
+                                mre_vectors=4, batch=True)
+    for k in range(args.hmc_therm):
+        seed = args.seed + 7000 + 100 * k
+        g.gaussian_d(p, seed + 1, 1.0)
+        g.gaussian_d(eta1, seed + 2, 0.5)
+        g.gaussian_d(eta2, seed + 3, 0.5)
+        out = g.hmc_trajectory_d(therm, p, eta1, eta2, accept_random=-1.0)
+        log("hmc thermalisation %d: dH %.4f plaquette %.6f, %.1f s"
+            % (k, out["h_new"] - out["h_old"], out["plaquette"], out["seconds"]))
+    g.gauge_backup(1)
     runs = {}
     for name, prec, batch in (("mixed16_batched", "f16", True), ("fp64", "f64", False)):
         g.gauge_restore(1)
@@ -268,7 +297,8 @@ def hmc(args, torch, dev, stream):
            "lattice": list(L), "beta": beta, "kappa": kappa, "csw": csw, "rho": rho,
            "integrator": "Omelyan (2MN), tau 1, %d outer steps, %d gauge steps each"
                          % (args.hmc_steps, args.hmc_mscale),
-           "trajectories": args.hmc, "runs": runs,
+           "trajectories": args.hmc, "thermalisation": "%d force-accepted trajectories of length 0.5"
+                                                          % args.hmc_therm, "runs": runs,
            "speedup_vs_fp64": runs["fp64"]["s_per_trajectory"] / runs["mixed16_batched"]["s_per_trajectory"]}
     g.close()
     return res
@@ -467,6 +497,7 @@ def main():
         batch = batched(args, torch, dev, stream)
 
     cg_bytes = 10560 * vh * (fp64.get("iters") or iters[0])
+    traffic, traffic_src = measured_traffic(L)
     fp64_ms = fp64.get("ms", ms)
     line = {
         "metric": "CG time to |r|/|b|<=1e-10: mixed+batched vs FP64 GPU; D-slash HBM GB/s",
@@ -501,7 +532,8 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": ds_gbs / HBM_PEAK_GBS,
-            "traffic": None,
+            "traffic": traffic,
+            "traffic_source": traffic_src,
             "kernel": "k_dslash<SF64, false, EPI_PLAIN> (FP64 hopping matrix, one parity)",
             "bytes_per_launch": ds_bytes,
         },
