@@ -155,7 +155,7 @@ int reduce_grid(const Context *c, int n) {
 int cg_start(Context *c, double2 *r, double2 *p, const double2 *b, CgState *st) {
   const int vh = c->g.vh, grid = nblocks(vh);
   k_cg_start<<<grid, kBlock, 0, c->stream>>>(r, p, b, vh, c->red);
-  k_finalize<1><<<1, 256, 0, c->stream>>>(c->red, grid, nullptr, FinCgStart{st});
+  k_finalize<1><<<1, kFinThreads, 0, c->stream>>>(c->red, grid, nullptr, FinCgStart{st});
   HB_HIP(hipGetLastError());
   return HB_OK;
 }
@@ -163,7 +163,7 @@ int cg_start(Context *c, double2 *r, double2 *p, const double2 *b, CgState *st) 
 int cg_update(Context *c, double2 *x, double2 *r, const double2 *p, const double2 *q, CgState *st) {
   const int vh = c->g.vh, grid = nblocks(vh);
   k_cg_update<<<grid, kBlock, 0, c->stream>>>(x, r, p, q, vh, st, c->red);
-  k_finalize<1><<<1, 256, 0, c->stream>>>(c->red, grid, &st->done, FinCgUpdate{st});
+  k_finalize<1><<<1, kFinThreads, 0, c->stream>>>(c->red, grid, &st->done, FinCgUpdate{st});
   HB_HIP(hipGetLastError());
   return HB_OK;
 }
@@ -178,7 +178,7 @@ int cg_pupdate(Context *c, double2 *p, const double2 *r, const CgState *st) {
 int residual(Context *c, double2 *out, const double2 *b, double *res_d) {
   const int vh = c->g.vh, grid = nblocks(vh);
   k_residual<<<grid, kBlock, 0, c->stream>>>(out, b, vh, c->red);
-  k_finalize<1><<<1, 256, 0, c->stream>>>(c->red, grid, nullptr, FinStore1{res_d});
+  k_finalize<1><<<1, kFinThreads, 0, c->stream>>>(c->red, grid, nullptr, FinStore1{res_d});
   HB_HIP(hipGetLastError());
   return HB_OK;
 }
@@ -186,7 +186,7 @@ int residual(Context *c, double2 *out, const double2 *b, double *res_d) {
 int dot_norm(Context *c, const double2 *x, const double2 *y, double *res_d) {
   const int vh = c->g.vh, grid = nblocks(vh);
   k_dot_norm<<<grid, kBlock, 0, c->stream>>>(x, y, vh, c->red);
-  k_finalize<2><<<1, 256, 0, c->stream>>>(c->red, grid, nullptr, FinStore2{res_d});
+  k_finalize<2><<<1, kFinThreads, 0, c->stream>>>(c->red, grid, nullptr, FinStore2{res_d});
   HB_HIP(hipGetLastError());
   return HB_OK;
 }

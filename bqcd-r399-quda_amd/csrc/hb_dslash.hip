@@ -115,7 +115,7 @@ int launch_dslash(const DslashArgs<S, G> &a, bool dag, int epi, hipStream_t st) 
   }
 #undef HB_DS
   if (epi == EPI_MTDOT)
-    k_finalize<1><<<1, 256, 0, st>>>(a.partials, (int)grid.x, a.skip, FinStore1{a.dot_out});
+    k_finalize<1><<<1, kFinThreads, 0, st>>>(a.partials, (int)grid.x, a.skip, FinStore1{a.dot_out});
   HB_HIP(hipGetLastError());
   return HB_OK;
 }

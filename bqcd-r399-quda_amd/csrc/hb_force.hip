@@ -453,7 +453,7 @@ int clover_action_dev(Context *c, double *res_d) {
   HB_CHECK(c->have_clover[HB_CLOVER_B], HB_E_STATE, "clover_action: clover B field not present");
   const int vh = c->g.vh;
   k_clover_action<<<nblocks(vh), kBlock, 0, c->stream>>>(c->clov[HB_CLOVER_B], 1, vh, c->red);
-  k_finalize<1><<<1, 256, 0, c->stream>>>(c->red, nblocks(vh), nullptr, FinTwice{res_d});
+  k_finalize<1><<<1, kFinThreads, 0, c->stream>>>(c->red, nblocks(vh), nullptr, FinTwice{res_d});
   HB_HIP(hipGetLastError());
   return HB_OK;
 }

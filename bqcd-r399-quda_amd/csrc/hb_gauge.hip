@@ -283,7 +283,7 @@ int build_clover_dev(Context *c, double csw_kappa) {
 int plaquette_dev(Context *c, double *res_d) {
   int n = 2 * c->g.vh;
   k_plaquette<<<nblocks(n), kBlock, 0, c->stream>>>(c->gauge, c->g, c->red);
-  k_finalize<1><<<1, 256, 0, c->stream>>>(c->red, nblocks(n), nullptr, FinStore1{res_d});
+  k_finalize<1><<<1, kFinThreads, 0, c->stream>>>(c->red, nblocks(n), nullptr, FinStore1{res_d});
   HB_HIP(hipGetLastError());
   return HB_OK;
 }

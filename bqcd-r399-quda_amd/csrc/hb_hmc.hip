@@ -121,7 +121,7 @@ struct Vec {
   int cdot(const double2 *x, const double2 *y, std::complex<double> *out) {
     const int g = grid_for(n);
     k_cdot<<<g, kBlock, 0, c->stream>>>(x, y, n, c->red);
-    k_finalize<2><<<1, 256, 0, c->stream>>>(c->red, g, nullptr, FinCopy2{res_d});
+    k_finalize<2><<<1, kFinThreads, 0, c->stream>>>(c->red, g, nullptr, FinCopy2{res_d});
     HB_HIP(hipGetLastError());
     HB_HIP(hipMemcpyAsync(host, res_d, 2 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     HB_HIP(hipStreamSynchronize(c->stream));
@@ -137,7 +137,7 @@ struct Vec {
   int sumsq(const double *x, size_t m, double *out) {
     const int g = grid_for(m);
     k_sumsq<<<g, kBlock, 0, c->stream>>>(x, m, c->red);
-    k_finalize<1><<<1, 256, 0, c->stream>>>(c->red, g, nullptr, FinStore1{res_d});
+    k_finalize<1><<<1, kFinThreads, 0, c->stream>>>(c->red, g, nullptr, FinStore1{res_d});
     HB_HIP(hipGetLastError());
     HB_HIP(hipMemcpyAsync(host, res_d, sizeof(double), hipMemcpyDeviceToHost, c->stream));
     HB_HIP(hipStreamSynchronize(c->stream));

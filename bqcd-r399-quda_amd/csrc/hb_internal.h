@@ -227,24 +227,39 @@ __device__ __forceinline__ void block_partials(const double (&v)[NV], double *pa
 }
 
 // one block: sums nparts partials per value (fixed order) and calls f(sums)
-// from thread 0; `skip` gates it like the producing kernel
+// from thread 0; `skip` gates it like the producing kernel.  1024 threads
+// with four independent loads in flight each: the partials come from L2, and
+// a single workgroup is latency bound, not bandwidth bound.
+constexpr int kFinThreads = 1024;
+
 template <int NV, class F>
-__global__ __launch_bounds__(256) void k_finalize(const double *__restrict__ partials, int nparts,
-                                                  const int *skip, F f) {
+__global__ __launch_bounds__(kFinThreads) void k_finalize(const double *__restrict__ partials, int nparts,
+                                                          const int *skip, F f) {
   if (skip && *skip) return;
-  __shared__ double lds[NV][4];
-  double acc[NV];
+  __shared__ double lds[NV][kFinThreads / 64];
+  constexpr int U = 4;
 #pragma unroll
   for (int k = 0; k < NV; k++) {
-    double s = 0.0;
-    for (int b = threadIdx.x; b < nparts; b += blockDim.x) s += partials[(size_t)k * nparts + b];
-    s = wave_sum(s);
+    const double *p = partials + (size_t)k * nparts;
+    double a[U] = {0.0, 0.0, 0.0, 0.0};
+    int b = threadIdx.x;
+    for (; b + (U - 1) * kFinThreads < nparts; b += U * kFinThreads) {
+#pragma unroll
+      for (int u = 0; u < U; u++) a[u] += p[b + u * kFinThreads];
+    }
+    for (; b < nparts; b += kFinThreads) a[0] += p[b];
+    double s = wave_sum((a[0] + a[1]) + (a[2] + a[3]));
     if ((threadIdx.x & 63) == 0) lds[k][threadIdx.x >> 6] = s;
   }
   __syncthreads();
   if (threadIdx.x == 0) {
+    double acc[NV];
 #pragma unroll
-    for (int k = 0; k < NV; k++) acc[k] = lds[k][0] + lds[k][1] + lds[k][2] + lds[k][3];
+    for (int k = 0; k < NV; k++) {
+      double s = 0.0;
+      for (int w = 0; w < kFinThreads / 64; w++) s += lds[k][w];
+      acc[k] = s;
+    }
     f(acc);
   }
 }

@@ -370,7 +370,7 @@ int cg_mixed_impl(Context *c, int op, double kappa, double rho, double2 *x, cons
     }
     HB_TRY(apply_f64(c, op, kappa, rho, ax, x, t64, y64, nullptr, nullptr));
     k_mix_true_residual<R><<<nblocks(vh), kBlock, 0, c->stream>>>(ax, b, r, vh, c->red);
-    k_finalize<1><<<1, 256, 0, c->stream>>>(c->red, nblocks(vh), nullptr, FinMixTrue{st, first});
+    k_finalize<1><<<1, kFinThreads, 0, c->stream>>>(c->red, nblocks(vh), nullptr, FinMixTrue{st, first});
     HB_HIP(hipGetLastError());
     k_mix_prestart<S, R><<<nblocks(vh), kBlock, 0, c->stream>>>(p, r, vh, st);
     HB_HIP(hipGetLastError());
@@ -385,7 +385,7 @@ int cg_mixed_impl(Context *c, int op, double kappa, double rho, double2 *x, cons
       HB_TRY((mtil_lp<S, G>(c, false, kappa, r_op, y, p, t, nullptr, SpinorPtr<S>{}, skip)));
       HB_TRY((mtil_lp<S, G>(c, true, kappa, r_op, q, y, t, &st->pap, p, skip)));
       k_mix_update<S, R><<<nblocks(vh), kBlock, 0, c->stream>>>(xlo, r, p, q, vh, st, c->red);
-      k_finalize<1><<<1, 256, 0, c->stream>>>(c->red, nblocks(vh), &st->flag, FinMixUpdate{st});
+      k_finalize<1><<<1, kFinThreads, 0, c->stream>>>(c->red, nblocks(vh), &st->flag, FinMixUpdate{st});
       HB_HIP(hipGetLastError());
       k_mix_pupdate<S, R><<<nblocks(vh), kBlock, 0, c->stream>>>(p, r, vh, st);
       HB_HIP(hipGetLastError());

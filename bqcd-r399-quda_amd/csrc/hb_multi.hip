@@ -424,7 +424,7 @@ int launch_multi_dslash(Context *c, const MultiArgs<S, G> &a, bool dag, int epi)
   }
 #undef HB_DM
   if (epi == EPI_MTDOT)
-    k_finalize<NR><<<1, 256, 0, c->stream>>>(a.partials, (int)grid.x, nullptr,
+    k_finalize<NR><<<1, kFinThreads, 0, c->stream>>>(a.partials, (int)grid.x, nullptr,
                                             FinMultiStore<NR>{a.dot_out, a.flag});
   HB_HIP(hipGetLastError());
   return HB_OK;
@@ -579,7 +579,7 @@ int cg_multi_block(Context *c, const MultiPlan &plan, const MultiFields<S, G> &f
     c->have_clover[HB_CLOVER_AINV] = hi;
     HB_TRY(rr);
     k_multi_true_residual<S, NR><<<nblocks(vh), kBlock, 0, c->stream>>>(ax, b[idx[k]], r, vh, k, c->red);
-    k_finalize<1><<<1, 256, 0, c->stream>>>(c->red, nblocks(vh), nullptr, FinMultiTrue{st, k, first});
+    k_finalize<1><<<1, kFinThreads, 0, c->stream>>>(c->red, nblocks(vh), nullptr, FinMultiTrue{st, k, first});
     k_multi_prestart<S, NR><<<nblocks(vh), kBlock, 0, c->stream>>>(p, r, vh, k, st);
     HB_HIP(hipGetLastError());
     return HB_OK;
@@ -591,7 +591,7 @@ int cg_multi_block(Context *c, const MultiPlan &plan, const MultiFields<S, G> &f
     HB_TRY((multi_apply<S, G, NR>(c, f, plan.ngroups, grp, kap, rh, q, p, t, y, st, plan.clover)));
     const int grid = nblocks(vh * NR);
     k_multi_update<S, NR><<<grid, kBlock, 0, c->stream>>>(xlo, r, p, q, vh, st, part);
-    k_finalize<NR><<<1, 256, 0, c->stream>>>(part, grid, nullptr, FinMultiUpdate<NR>{st});
+    k_finalize<NR><<<1, kFinThreads, 0, c->stream>>>(part, grid, nullptr, FinMultiUpdate<NR>{st});
     k_multi_pupdate<S, NR><<<grid, kBlock, 0, c->stream>>>(p, r, vh, st);
     HB_HIP(hipGetLastError());
     return HB_OK;
