@@ -251,7 +251,7 @@ def hmc(args, torch, dev, stream):
     therm = bqcd_hip.hmc_params(beta=beta, kappa=kappa, csw=csw, rho=rho, tau=0.5 / args.hmc_steps,
                                 ntau=args.hmc_steps, m_scale=args.hmc_mscale, model="B",
                                 integrator="omelyan", rest=rest, maxiter=50000, precision="f16",
-                                mre_vectors=4, batch=True)
+                                mre_vectors=4, batch=True, verbose=True)
     for k in range(args.hmc_therm):
         seed = args.seed + 7000 + 100 * k
         g.gaussian_d(p, seed + 1, 1.0)
@@ -267,7 +267,7 @@ def hmc(args, torch, dev, stream):
         prm = bqcd_hip.hmc_params(beta=beta, kappa=kappa, csw=csw, rho=rho, tau=1.0 / args.hmc_steps,
                                   ntau=args.hmc_steps, m_scale=args.hmc_mscale, model="B",
                                   integrator="omelyan", rest=rest, maxiter=50000, precision=prec,
-                                  mre_vectors=4, batch=batch)
+                                  mre_vectors=4, batch=batch, verbose=True)
         trajs = []
         for k in range(args.hmc):
             seed = args.seed + 100 * k

@@ -43,7 +43,7 @@ class HmcParams(ctypes.Structure):
                 ("m_scale", ctypes.c_int), ("model", ctypes.c_char), ("integrator", ctypes.c_int),
                 ("lam", ctypes.c_double), ("rest", ctypes.c_float), ("maxiter", ctypes.c_int),
                 ("precision", ctypes.c_int), ("delta", ctypes.c_double), ("max_inner", ctypes.c_int),
-                ("mre_vectors", ctypes.c_int), ("batch", ctypes.c_int)]
+                ("mre_vectors", ctypes.c_int), ("batch", ctypes.c_int), ("verbose", ctypes.c_int)]
 
 
 class HmcOut(ctypes.Structure):
@@ -63,9 +63,10 @@ INTEGRATOR = {"leapfrog": 0, "omelyan": 1}
 
 def hmc_params(beta, kappa, csw, rho, tau, ntau, m_scale=1, model="C", integrator="leapfrog",
                lam=0.0, rest=1e-20, maxiter=10000, precision="f64", delta=0.1, max_inner=1000,
-               mre_vectors=0, batch=False):
+               mre_vectors=0, batch=False, verbose=False):
     return HmcParams(beta, kappa, csw, rho, tau, ntau, m_scale, model.encode(), INTEGRATOR[integrator],
-                     lam, rest, maxiter, PREC[precision], delta, max_inner, mre_vectors, int(batch))
+                     lam, rest, maxiter, PREC[precision], delta, max_inner, mre_vectors, int(batch),
+                     int(verbose))
 
 
 class NoConvergence(HipError):

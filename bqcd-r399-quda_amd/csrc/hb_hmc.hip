@@ -170,6 +170,15 @@ struct Traj {
   double sf1 = 0.0, sf2 = 0.0;
   int ncall = 0, niter_tot = 0, niter_max = 0;
   int n_ir = 0, n_uv = 0;
+  std::chrono::steady_clock::time_point t0;
+
+  void progress(int it) {
+    if (!prm.verbose) return;
+    const double s = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    fprintf(stderr, "[hb_hmc] outer step %d/%d: %d solves, %d CG iterations, %.1f s\n", it, prm.ntau,
+            ncall, niter_tot, s);
+    fflush(stderr);
+  }
 
   // cg_stat bookkeeping (cg.F90:126-131)
   void count(int it) {
@@ -410,6 +419,7 @@ struct Traj {
         HB_TRY(p_uv(s_uv, calc));
       }
       HB_TRY(p_ir(s_ir, calc));
+      progress(it);
     }
     return HB_OK;
   }
@@ -443,6 +453,7 @@ struct Traj {
           pending_uv = l * e;
         }
       }
+      progress(it);
     }
     return HB_OK;
   }
@@ -495,6 +506,7 @@ int trajectory(Context *c, const hb_hmc_params *prm, const double *p_d, const do
   Traj T{};
   T.c = c;
   T.prm = *prm;
+  T.t0 = t0;
   if (T.prm.delta <= 0.0) T.prm.delta = 0.1;
   if (T.prm.max_inner <= 0) T.prm.max_inner = 1000;
   T.ck = prm->csw * prm->kappa;

@@ -208,6 +208,7 @@ typedef struct {
   int mre_vectors;                /* chronological guesses (mre.F90); 0: the source */
   int batch;                      /* model B: solve the two pseudofermions together
                                      (multi-RHS solver, precision F32/F16) */
+  int verbose;                    /* 1: one progress line per outer step on stderr */
 } hb_hmc_params;
 
 typedef struct {
