@@ -116,6 +116,11 @@ def lib():
         L.hb_hmc_trajectory_d.argtypes = [_vp, ctypes.POINTER(HmcParams), _vp, _vp, _vp, ctypes.c_double,
                                           ctypes.POINTER(HmcOut)]
         L.hb_gaussian_d.argtypes = [_vp, ctypes.c_ulonglong, ctypes.c_double, _vp, ctypes.c_longlong]
+        L.hb_conf_write.argtypes = [ctypes.c_int * 4, ctypes.c_int * 4, _dp, ctypes.c_int, ctypes.c_char_p,
+                                    ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double,
+                                    ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                                    ctypes.c_char_p, ctypes.c_int]
+        L.hb_conf_read.argtypes = [ctypes.c_char_p, ctypes.c_int * 4, _dp, ctypes.c_int, _dp]
         L.hb_build_clover.argtypes = [_vp, ctypes.c_double]
         L.hb_plaquette.argtypes = [_vp, _dp]
         L.hb_gauge_random.argtypes = [_vp, ctypes.c_ulonglong, ctypes.c_double]
@@ -166,6 +171,28 @@ def _p(a):
 def _tp(t):
     """device pointer of a torch tensor"""
     return ctypes.c_void_p(t.data_ptr())
+
+
+def conf_write(L, u, directory, run=1, traj=0, ensemble=1, former=1, beta=0.0, kappa=0.0, csw=0.0,
+               h=0.0, plaq_energy=0.0, bc=(1, 1, 1, -1)):
+    """conf_write (conf.F90:384): BQCD configuration files of the host gauge
+    field u (BQCD layout, physical links); returns the .info path.  Host only."""
+    u = np.ascontiguousarray(u, dtype=np.float64)
+    out = ctypes.create_string_buffer(1024)
+    _check(lib().hb_conf_write((ctypes.c_int * 4)(*L), (ctypes.c_int * 4)(*bc), _p(u), u.shape[2],
+                               directory.encode(), run, traj, ensemble, former, beta, kappa, csw, h,
+                               plaq_energy, out, len(out)))
+    return out.value.decode()
+
+
+def conf_read(info_file, L):
+    """conf_read (conf.F90:305): the gauge field (BQCD layout) and the header's
+    plaquette energy of a BQCD configuration; checksums are verified.  Host only."""
+    vh = L[0] * L[1] * L[2] * L[3] // 2
+    u = np.zeros((4, 2, vh, 3, 3, 2))
+    plaq = ctypes.c_double(0.0)
+    _check(lib().hb_conf_read(info_file.encode(), (ctypes.c_int * 4)(*L), _p(u), vh, ctypes.byref(plaq)))
+    return u, plaq.value
 
 
 class Lattice:

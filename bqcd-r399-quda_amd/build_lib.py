@@ -21,7 +21,7 @@ ARCH = "gfx950"
 
 SOURCES = ["hb_context.cpp", "hb_fields.hip", "hb_dslash.hip", "hb_blas.hip", "hb_cg.hip",
            "hb_gauge.hip", "hb_mixed.hip", "hb_multi.hip",
-           "hb_force.hip", "hb_hmc.hip"]
+           "hb_force.hip", "hb_hmc.hip", "hb_conf.cpp"]
 
 FLAGS = [
     "-O3", "-std=c++17", "-fPIC", "--offload-arch=" + ARCH,

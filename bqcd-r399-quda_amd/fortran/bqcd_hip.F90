@@ -23,6 +23,27 @@ module bqcd_hip
   integer(c_int), parameter :: HB_CLOVER_A = 0, HB_CLOVER_AINV = 1, HB_CLOVER_B = 2
   integer(c_int), parameter :: HB_PREC_F64 = 64, HB_PREC_F32 = 32, HB_PREC_F16 = 16, &
                                HB_PREC_F16_GAUGE = 18
+  integer(c_int), parameter :: HB_INT_LEAPFROG = 0, HB_INT_OMELYAN = 1
+
+  ! hb_hmc_params / hb_hmc_out of include/bqcd_hip.h
+  type, bind(C) :: hb_hmc_params
+     real(c_double)         :: beta, kappa, csw, rho, tau
+     integer(c_int)         :: ntau, m_scale
+     character(kind=c_char) :: model
+     integer(c_int)         :: integrator
+     real(c_double)         :: lambda
+     real(c_float)          :: rest
+     integer(c_int)         :: maxiter, precision
+     real(c_double)         :: delta
+     integer(c_int)         :: max_inner, mre_vectors, batch, verbose
+  end type hb_hmc_params
+
+  type, bind(C) :: hb_hmc_out
+     real(c_double) :: exp_dh, h_old, h_new, sg_old, sg_new, sp_old, sp_new, sd_old, sd_new
+     real(c_double) :: sf1_old, sf1_new, sf2_old, sf2_new, plaquette
+     integer(c_int) :: accepted, cg_ncall, cg_niter_tot, cg_niter_max, n_force_ir, n_force_uv
+     real(c_double) :: seconds
+  end type hb_hmc_out
 
   interface
 
@@ -208,6 +229,42 @@ module bqcd_hip
        import :: c_int, c_ptr
        type(c_ptr), value    :: ctx
        integer(c_int), value :: slot
+     end function
+
+     ! one HMC trajectory (hmc.F90:28-180); p, eta1, eta2 in BQCD layouts
+     integer(c_int) function hb_hmc_trajectory(ctx, prm, p, site_stride, eta1, eta2, accept_random, out) &
+          bind(C, name="hb_hmc_trajectory")
+       import :: c_int, c_ptr, c_double, hb_hmc_params, hb_hmc_out
+       type(c_ptr), value            :: ctx
+       type(hb_hmc_params), intent(in) :: prm
+       type(*), intent(in)           :: p(*)
+       integer(c_int), value         :: site_stride
+       type(*), intent(in)           :: eta1(*)
+       type(*), intent(in), optional :: eta2(*)
+       real(c_double), value         :: accept_random
+       type(hb_hmc_out), intent(out) :: out
+     end function
+
+     ! BQCD configuration files (conf.F90:305-470), host only; strings end in c_null_char
+     integer(c_int) function hb_conf_write(L, bc, u, site_stride, dir, run, traj, ensemble, former, &
+          beta, kappa, csw, h, plaq_energy, info_path, info_path_len) bind(C, name="hb_conf_write")
+       import :: c_int, c_double, c_char
+       integer(c_int), intent(in)         :: L(4), bc(4)
+       type(*), intent(in)                :: u(*)
+       integer(c_int), value              :: site_stride, run, traj, ensemble, former, info_path_len
+       character(kind=c_char), intent(in) :: dir(*)
+       real(c_double), value              :: beta, kappa, csw, h, plaq_energy
+       character(kind=c_char)             :: info_path(*)
+     end function
+
+     integer(c_int) function hb_conf_read(info_file, L, u, site_stride, plaq_energy) &
+          bind(C, name="hb_conf_read")
+       import :: c_int, c_double, c_char
+       character(kind=c_char), intent(in)    :: info_file(*)
+       integer(c_int), intent(in)            :: L(4)
+       type(*)                               :: u(*)
+       integer(c_int), value                 :: site_stride
+       real(c_double), intent(out), optional :: plaq_energy
      end function
 
      integer(c_int) function hb_last_timing(ctx, solve_ms, n_matvec) bind(C, name="hb_last_timing")

@@ -13,6 +13,7 @@
 !   hip_dsd(...)               dsd                            (dsd.F90:28-49)
 !   hip_dsg(...)               dsg                            (dsg.F90:28-63)
 !   hip_hmc_u(...)             hmc_u                          (hmc_u.F90:28-69)
+!   hip_hmc(...)               hmc (one trajectory)           (hmc.F90:28-180)
 !
 ! Conventions kept from the reference:
 !  - hip_solver and hip_dsf are called with the links bc-flipped (flip_bc
@@ -278,4 +279,96 @@ subroutine hip_hmc_u(p, conf, step, para)
      call hip_check(hb_build_clover(hip_ctx, real(para%csw_kappa, c_double)), "hb_build_clover")
      call hip_check(hb_save_clover(hip_ctx, conf%a, conf%i, conf%b), "hb_save_clover")
   endif
+end
+
+!-------------------------------------------------------------------------------
+subroutine hip_hmc(para, conf, out, force_accept, test)
+
+  ! one trajectory of hmc() (hmc.F90:28-180) on the device: the momenta and
+  ! pseudofermion noise are drawn here with BQCD's own generator in hmc()'s
+  ! order (hmc_init_p, then hmc_init_phi's ran_gauss_volh), the acceptance
+  ! number likewise after them, so with the same seed this reproduces hmc();
+  ! the leapfrog and force groups follow para%model.  conf%u and the clover
+  ! term come back updated (or unchanged on rejection); conf%phi is not set.
+
+  use module_hip_solver
+  use module_cg
+  use module_mre
+  use module_p_interface
+  use module_switches
+  use module_vol
+  use typedef_hmc
+  implicit none
+
+  type(hmc_para), intent(in)    :: para
+  type(hmc_conf), intent(inout) :: conf
+  type(hmc_out),  intent(out)   :: out
+  integer,        intent(in)    :: force_accept
+  integer,        intent(in)    :: test
+
+  P_GENERATOR_FIELD, save :: p
+  P_SPINCOL_FIELD, save   :: eta1, eta2
+  type(hb_hmc_params)     :: prm
+  type(hb_hmc_out)        :: o
+  real(c_double)          :: r
+  REAL, external          :: ranf
+
+  if (test /= 0) call die("hip_hmc(): reversibility tests run with hmc()")
+  if (para%h /= ZERO) call die("hip_hmc(): h /= 0 is not supported")
+  if (.not. c_associated(hip_ctx)) call die("hip_hmc(): init_hip_solver() not called")
+
+  if (.not. associated(p)) then
+     call allocate_gen_field(p)
+     call allocate_sc_field(eta1)
+     call allocate_sc_field(eta2)
+  endif
+
+  call hmc_init_p(p)
+  call ran_gauss_volh(NDIRAC * NCOL, eta1, HALF, EVEN)
+  if (switches%hasenbusch) call ran_gauss_volh(NDIRAC * NCOL, eta2, HALF, EVEN)
+
+  r = -1.0_c_double
+  if (force_accept == 0) r = ranf()
+
+  prm%beta = para%beta
+  prm%kappa = para%kappa
+  prm%csw = para%csw
+  if (.not. switches%clover) prm%csw = 0.0_c_double
+  prm%rho = para%rho
+  prm%tau = para%tau
+  prm%ntau = para%ntau
+  prm%m_scale = para%m_scale
+  prm%model = para%model
+  prm%integrator = HB_INT_LEAPFROG
+  prm%lambda = 0.0_c_double
+  prm%rest = real(cg_para%rest, c_float)
+  prm%maxiter = cg_para%maxiter
+  prm%precision = hip_precision
+  prm%delta = hip_delta
+  prm%max_inner = hip_max_inner
+  prm%mre_vectors = mre_n_vec
+  prm%batch = 0
+  prm%verbose = 0
+
+  call hip_check(hb_load_gauge(hip_ctx, conf%u, int(volh_tot, c_int), 0_c_int), "hb_load_gauge")
+  if (switches%hasenbusch) then
+     call hip_check(hb_hmc_trajectory(hip_ctx, prm, p, int(volh_tot, c_int), eta1, eta2, r, o), "hip_hmc()")
+  else
+     call hip_check(hb_hmc_trajectory(hip_ctx, prm, p, int(volh_tot, c_int), eta1, accept_random=r, out=o), &
+                    "hip_hmc()")
+  endif
+
+  call hip_check(hb_save_gauge(hip_ctx, conf%u, int(volh_tot, c_int), 0_c_int), "hb_save_gauge")
+  call xbound_g_field(conf%u)
+  if (switches%clover) then
+     call hip_check(hb_save_clover(hip_ctx, conf%a, conf%i, conf%b), "hb_save_clover")
+  endif
+
+  out%exp_dh = o%exp_dh
+  out%accepted = o%accepted
+  out%sg = o%sg_new
+  out%sf = o%sf1_new
+  out%cg_ncall = o%cg_ncall
+  out%cg_niter_max = o%cg_niter_max
+  out%cg_niter_tot = o%cg_niter_tot
 end

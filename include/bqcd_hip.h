@@ -237,6 +237,23 @@ int hb_hmc_trajectory(hb_context *ctx, const hb_hmc_params *prm, const double *p
 int hb_hmc_trajectory_d(hb_context *ctx, const hb_hmc_params *prm, const double *p_d,
                         const double *eta1_d, const double *eta2_d, double accept_random,
                         hb_hmc_out *out);
+/* ---- BQCD configuration files (host only, no device needed) -----------
+ * conf_write / conf_read (conf.F90:305-470) with field_io (comm/field_io_mpi.F90)
+ * and the ConfInfoHeader / CheckSum blocks (conf_info.F90): one big-endian file
+ * per time slice holding the first two columns of every link, a .info file
+ * with the lattice, parameters and the POSIX cksum of each slice file.  u is a
+ * host gauge field in BQCD layout (physical links).  Files are named as
+ * files.F90 does: bqcd.<run>.<ensemble>.<former>.<traj>.<t>.u and .info in dir;
+ * info_path (may be NULL) receives the .info path. */
+int hb_conf_write(const int L[4], const int bc[4], const double *u, int site_stride, const char *dir,
+                  int run, int traj, int ensemble, int former, double beta, double kappa, double csw,
+                  double h, double plaq_energy, char *info_path, int info_path_len);
+/* reads the configuration an .info file describes (slice files relative to the
+ * .info file's directory), checking the format, the lattice and every slice's
+ * checksum; the third column of each link is rebuilt (u_complete,
+ * su3.F90:301-310).  plaq_energy (may be NULL) receives the header's value. */
+int hb_conf_read(const char *info_file, const int L[4], double *u, int site_stride, double *plaq_energy);
+
 /* n Gaussian numbers of variance var on the device (seeded, counter-based):
  * momenta are var 1 (hmc_init_p), pseudofermion noise var 1/2 per real
  * component (ran_gauss_volh(12, ., 1/2, .)) */

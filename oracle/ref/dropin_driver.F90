@@ -9,6 +9,9 @@
 ! stdin (as golden_driver): Lx Ly Lz Lt / kappa csw rho / rest maxiter /
 ! step beta / input directory (u.bin, eta_e.bin, mom.bin) / (unused) / 0
 !
+! The last comparison is a whole trajectory: hmc() against hip_hmc() after
+! the same ranset (model C with rho, model A without).
+!
 ! Prints one "DROPIN name value" line per comparison (iteration counts, and
 ! max |hip - reference| / max |reference| of each result) and exits with
 ! status 1 if any comparison fails its bound.
@@ -25,11 +28,13 @@ program dropin_driver
   use module_switches
   use module_p_interface
   use module_hip_solver
+  use module_mre
   implicit none
 
   type(type_para) :: para
   type(hmc_para)  :: hp
   type(hmc_conf)  :: conf, conf2
+  type(hmc_out)   :: out_ref, out_hip
   integer         :: lat(4), maxiter, bench_iters, it_ref, it_hip, mu, eo, nfail
   REAL            :: kappa, csw, rho, step, beta
   real            :: rest
@@ -59,6 +64,7 @@ program dropin_driver
   call init_xbound()
   call init_cg_para(rest, maxiter, 1)
   call init_cg_stat()
+  mre_n_vec = 0
 
   switches%clover = .true.
   switches%hasenbusch = .false.
@@ -71,6 +77,7 @@ program dropin_driver
   hp%rho = rho
 
   nullify(conf%u, conf%a, conf%i, conf%b, conf2%u, conf2%a, conf2%i, conf2%b)
+  nullify(conf%phi, conf%phi2, conf2%phi, conf2%phi2)
   nullify(p_ref, p_hip, mom)
   call allocate_g_field(conf%u)
   call allocate_clover_field_a(conf%a)
@@ -167,6 +174,33 @@ program dropin_driver
   call report(conf2%i, conf%i, 72 * 2 * volh, 'hmc_u_clover_i', 1.0e-12_8)
   call report(conf2%b, conf%b, 72 * 2 * volh, 'hmc_u_clover_b', 1.0e-12_8)
 
+  ! --- a whole trajectory: hmc() against hip_hmc(), same seed ----------------
+  hp%beta = beta
+  hp%rho = rho
+  hp%tau = 0.05_8
+  hp%ntau = 4
+  hp%m_scale = 2
+  hp%model = 'C'
+  if (rho == ZERO) hp%model = 'A'
+  switches%dynamical = .true.
+  switches%hasenbusch = (hp%model /= 'A')
+  call allocate_sc_field(conf%phi)
+  call allocate_sc_field(conf%phi2)
+  call init_cg_para(rest, maxiter, 0)
+  call copy_conf(conf2, conf)
+  call init_cg_stat()
+  call ranset(4711_8, 0_8)
+  call hmc(hp, conf, out_ref, 1, 0)
+  call ranset(4711_8, 0_8)
+  call hip_hmc(hp, conf2, out_hip, 1, 0)
+  call report_int('hmc_cg_iters_ref', out_ref%cg_niter_tot)
+  call report_int('hmc_cg_iters_hip', out_hip%cg_niter_tot)
+  if (out_ref%cg_niter_tot /= out_hip%cg_niter_tot) nfail = nfail + 1
+  call report_real('hmc_exp_dh_ref', out_ref%exp_dh)
+  call report_real('hmc_exp_dh_hip', out_hip%exp_dh)
+  if (.not. abs(out_hip%exp_dh - out_ref%exp_dh) <= 1.0e-9_8 * abs(out_ref%exp_dh)) nfail = nfail + 1
+  call report(conf2%u, conf%u, 2 * size(conf%u), 'hmc_links', 1.0e-10_8)
+
   call end_hip_solver()
 
   write(*, '(a, i4)') 'DROPIN failures ', nfail
@@ -181,6 +215,12 @@ CONTAINS
     dst%i = src%i
     dst%b = src%b
   end subroutine copy_conf
+
+  subroutine report_real(name, v)
+    character(len=*) :: name
+    real(8) :: v
+    write(*, '(a, a, a, es25.16)') 'DROPIN ', name, ' ', v
+  end subroutine report_real
 
   subroutine report_int(name, v)
     character(len=*) :: name
