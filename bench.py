@@ -68,10 +68,26 @@ def parse():
     ap.add_argument("--hmc-lattice", default="48,48,48,96")
     ap.add_argument("--hmc-kappa", type=float, default=0.1365)
     ap.add_argument("--hmc-rho", type=float, default=0.1)
-    ap.add_argument("--hmc-steps", type=int, default=20)
+    ap.add_argument("--hmc-steps", default="20",
+                    help="outer steps per trajectory; a comma list runs the trajectories once per "
+                         "step count from the same start (the step-size scaling of Delta H)")
     ap.add_argument("--hmc-mscale", type=int, default=2)
     ap.add_argument("--hmc-therm", type=int, default=3,
                     help="force-accepted trajectories from the smeared start before the measured ones")
+    ap.add_argument("--hmc-quench", type=int, default=0, metavar="NTRAJ",
+                    help="pure-gauge molecular-dynamics trajectories (tau 1, 20 steps, fresh momenta "
+                         "each, no accept/reject) at --hmc-quench-beta before the dynamical "
+                         "thermalisation: they give the smeared start equilibrium-like short-distance "
+                         "fluctuations at the cost of gauge forces only")
+    ap.add_argument("--hmc-quench-beta", type=float, default=5.7)
+    ap.add_argument("--hmc-therm-steps", type=int, default=20,
+                    help="outer steps of each thermalisation trajectory")
+    ap.add_argument("--hmc-first", type=int, default=0,
+                    help="index of the first measured trajectory (its draws and accept number): "
+                         "a run split over several jobs continues where the last one stopped")
+    ap.add_argument("--hmc-runs", default="mixed16_batched,fp64",
+                    help="solver runs of the trajectory workload, comma separated "
+                         "(mixed16_batched, mixed32_batched, fp64)")
     ap.add_argument("--scan", action="store_true",
                     help="print plaquette and FP64 iteration counts per smearing sweep, then exit")
     return ap.parse_args()
@@ -246,10 +262,25 @@ def hmc(args, torch, dev, stream):
     eta2 = torch.empty_like(eta1)
     rest = 1e-20 * 12 * vh  # |r|/|b| ~ 1e-10 for the unit-variance sources
     rng = __import__("random").Random(args.seed + 12)
+    log("hmc start: plaquette %.6f after %d smearing sweeps" % (g.plaquette(), args.sweeps))
+    if args.hmc_quench:
+        t0 = time.perf_counter()
+        dt, nstep = 0.05, 20
+        for k in range(args.hmc_quench):
+            # leapfrog of hmc_leap_frog.F90 with the gauge force only
+            g.gaussian_d(p, args.seed + 5000 + k, 1.0)
+            g.dsg_d(p, 0.5 * dt, args.hmc_quench_beta)
+            for s in range(nstep):
+                g.hmc_u_d(p, dt, 0.0)
+                g.dsg_d(p, dt if s < nstep - 1 else 0.5 * dt, args.hmc_quench_beta)
+            if k % 5 == 4 or k == args.hmc_quench - 1:
+                torch.cuda.synchronize()
+                log("hmc pure-gauge trajectory %d (beta %.3f): plaquette %.6f, %.1f s"
+                    % (k, args.hmc_quench_beta, g.plaquette(), time.perf_counter() - t0))
     # the smeared start is far from equilibrium at beta 5.29: force-accepted
     # trajectories (as the reference's hmc_accept_first) bring it closer
-    therm = bqcd_hip.hmc_params(beta=beta, kappa=kappa, csw=csw, rho=rho, tau=0.5 / args.hmc_steps,
-                                ntau=args.hmc_steps, m_scale=args.hmc_mscale, model="B",
+    therm = bqcd_hip.hmc_params(beta=beta, kappa=kappa, csw=csw, rho=rho, tau=0.5 / args.hmc_therm_steps,
+                                ntau=args.hmc_therm_steps, m_scale=args.hmc_mscale, model="B",
                                 integrator="omelyan", rest=rest, maxiter=50000, precision="f16",
                                 mre_vectors=4, batch=True, verbose=True)
     for k in range(args.hmc_therm):
@@ -262,29 +293,35 @@ def hmc(args, torch, dev, stream):
             % (k, out["h_new"] - out["h_old"], out["plaquette"], out["seconds"]))
     g.gauge_backup(1)
     runs = {}
-    for name, prec, batch in (("mixed16_batched", "f16", True), ("fp64", "f64", False)):
+    solvers = {"mixed16_batched": ("f16", True), "mixed32_batched": ("f32", True), "fp64": ("f64", False)}
+    # one accept/reject number per trajectory, shared by every run
+    accept_u = [rng.random() for _ in range(args.hmc_first + args.hmc)][args.hmc_first:]
+    steps = [int(v) for v in args.hmc_steps.split(",")]
+    for nsteps, name in [(n, r) for n in steps for r in args.hmc_runs.split(",")]:
+        prec, batch = solvers[name]
+        key = name if len(steps) == 1 else "%s@%d" % (name, nsteps)
         g.gauge_restore(1)
-        prm = bqcd_hip.hmc_params(beta=beta, kappa=kappa, csw=csw, rho=rho, tau=1.0 / args.hmc_steps,
-                                  ntau=args.hmc_steps, m_scale=args.hmc_mscale, model="B",
+        prm = bqcd_hip.hmc_params(beta=beta, kappa=kappa, csw=csw, rho=rho, tau=1.0 / nsteps,
+                                  ntau=nsteps, m_scale=args.hmc_mscale, model="B",
                                   integrator="omelyan", rest=rest, maxiter=50000, precision=prec,
                                   mre_vectors=4, batch=batch, verbose=True)
         trajs = []
         for k in range(args.hmc):
-            seed = args.seed + 100 * k
+            seed = args.seed + 100 * (args.hmc_first + k)
             g.gaussian_d(p, seed + 1, 1.0)
             g.gaussian_d(eta1, seed + 2, 0.5)
             g.gaussian_d(eta2, seed + 3, 0.5)
-            u = rng.random() if name == "mixed16_batched" else trajs_acc[k]
-            out = g.hmc_trajectory_d(prm, p, eta1, eta2, accept_random=u)
-            out["accept_random"] = u
+            out = g.hmc_trajectory_d(prm, p, eta1, eta2, accept_random=accept_u[k])
             log("hmc %s trajectory %d: dH %.6f exp(-dH) %.6f plaquette %.6f accepted %d, %d CG calls, "
-                "%d iterations, %.1f s" % (name, k, out["h_new"] - out["h_old"], out["exp_dh"],
+                "%d iterations, %.1f s" % (key, args.hmc_first + k, out["h_new"] - out["h_old"], out["exp_dh"],
                                            out["plaquette"], out["accepted"], out["cg_ncall"],
                                            out["cg_niter_tot"], out["seconds"]))
+            log("    dH terms: beta*sg %.4f sp %.4f sd %.4f sf1 %.4f sf2 %.4f"
+                % (beta * (out["sg_new"] - out["sg_old"]), out["sp_new"] - out["sp_old"],
+                   out["sd_new"] - out["sd_old"], out["sf1_new"] - out["sf1_old"],
+                   out["sf2_new"] - out["sf2_old"]))
             trajs.append(out)
-        if name == "mixed16_batched":
-            trajs_acc = [t["accept_random"] for t in trajs]
-        runs[name] = {
+        runs[key] = {
             "delta_h": [t["h_new"] - t["h_old"] for t in trajs],
             "plaquette": [t["plaquette"] for t in trajs],
             "acceptance": sum(t["accepted"] for t in trajs) / max(1, len(trajs)),
@@ -295,11 +332,15 @@ def hmc(args, torch, dev, stream):
     res = {"workload": "HMC trajectories, clover-Wilson Nf=2 with two Hasenbusch pseudofermions, %s"
                        % "x".join(map(str, L)),
            "lattice": list(L), "beta": beta, "kappa": kappa, "csw": csw, "rho": rho,
-           "integrator": "Omelyan (2MN), tau 1, %d outer steps, %d gauge steps each"
+           "integrator": "Omelyan (2MN), tau 1, %s outer steps, %d gauge steps each"
                          % (args.hmc_steps, args.hmc_mscale),
-           "trajectories": args.hmc, "thermalisation": "%d force-accepted trajectories of length 0.5"
-                                                          % args.hmc_therm, "runs": runs,
-           "speedup_vs_fp64": runs["fp64"]["s_per_trajectory"] / runs["mixed16_batched"]["s_per_trajectory"]}
+           "trajectories": args.hmc, "first_trajectory": args.hmc_first, "thermalisation": "%s%d force-accepted trajectories of length 0.5 in %d steps"
+                             % ("%d pure-gauge MD trajectories at beta %.3f, then " % (args.hmc_quench, args.hmc_quench_beta)
+                                if args.hmc_quench else "", args.hmc_therm, args.hmc_therm_steps), "runs": runs,
+           "accept_random": accept_u}
+    if "fp64" in runs and "mixed16_batched" in runs:
+        res["speedup_vs_fp64"] = (runs["fp64"]["s_per_trajectory"]
+                                  / runs["mixed16_batched"]["s_per_trajectory"])
     g.close()
     return res
 
