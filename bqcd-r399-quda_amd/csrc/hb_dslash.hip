@@ -35,17 +35,7 @@ __global__ __launch_bounds__(kBlock) void k_dslash(DslashArgs<S, G> a) {
     int xc[4];
     site_coords(g, e, i, xc[0], xc[1], xc[2], xc[3]);
     const SpinorView<S> in(a.in, vh);
-    // forward projector sign: (1 - gamma) for D, (1 + gamma) for D^+
-    constexpr int SF = DAG ? 1 : -1;
-    constexpr int SB = -SF;
-    hop<S, G, 3, true, SF>(acc, a.U, in, g, e, i, xc);  HB_SCHED_FENCE();
-    hop<S, G, 3, false, SB>(acc, a.U, in, g, e, i, xc); HB_SCHED_FENCE();
-    hop<S, G, 2, true, SF>(acc, a.U, in, g, e, i, xc);  HB_SCHED_FENCE();
-    hop<S, G, 2, false, SB>(acc, a.U, in, g, e, i, xc); HB_SCHED_FENCE();
-    hop<S, G, 1, true, SF>(acc, a.U, in, g, e, i, xc);  HB_SCHED_FENCE();
-    hop<S, G, 1, false, SB>(acc, a.U, in, g, e, i, xc); HB_SCHED_FENCE();
-    hop<S, G, 0, true, SF>(acc, a.U, in, g, e, i, xc);  HB_SCHED_FENCE();
-    hop<S, G, 0, false, SB>(acc, a.U, in, g, e, i, xc); HB_SCHED_FENCE();
+    hop_all<S, G, DAG>(acc, a.U, in, g, e, i, xc);
   }
 
   if constexpr (EPI == EPI_PLAIN) {

@@ -28,33 +28,60 @@ __device__ __forceinline__ void hop(typename S::V (&acc)[12], const typename G::
   }
   const int j = site_index(g, c[0], c[1], c[2], c[3]);
   V h[6], w[6];
-  if constexpr (MU == 3) {
-    V p[6];
-    in.load_pair(p, j * NR + r, SG < 0 ? 2 : 0);
-    const A f = (A)2 * sgn;
+  if constexpr (S::kNorm || link_raw<G>()) {
+    // 16-bit words are used as stored: the spinor's site factor, the link's
+    // 1/32767 and the boundary sign go into one multiply of the half spinor
+    A f;
+    if constexpr (MU == 3) {
+      V p[6];
+      f = (A)2 * sgn * in.load_pair_raw(p, j * NR + r, SG < 0 ? 2 : 0);
 #pragma unroll
-    for (int cc = 0; cc < 3; cc++) {
-      h[2 * cc] = cscale<V>(f, p[cc]);
-      h[2 * cc + 1] = cscale<V>(f, p[3 + cc]);
+      for (int cc = 0; cc < 3; cc++) {
+        h[2 * cc] = p[cc];
+        h[2 * cc + 1] = p[3 + cc];
+      }
+    } else {
+      V p[12];
+      f = sgn * in.load_raw(p, j * NR + r);
+      project<MU, SG>(h, p);
     }
-  } else {
-    V p[12];
-    in.load(p, j * NR + r);
-    project<MU, SG>(h, p);
-    if (sgn < (A)0) {
+    V u[9];
+    if (FWD) f *= load_link_f<G>(u, U, MU, e, i, vh);
+    else f *= load_link_f<G>(u, U, MU, 1 - e, j, vh);
 #pragma unroll
-      for (int k = 0; k < 6; k++) h[k] = cmk<V>(-h[k].x, -h[k].y);
-    }
-  }
-  V u[9];
-  if (FWD) {
-    load_link<G>(u, U, MU, e, i, vh);
-    su3_mul_half(w, u, h);
+    for (int k = 0; k < 6; k++) h[k] = cscale<V>(f, h[k]);
+    if (FWD) su3_mul_half(w, u, h);
+    else su3_adj_mul_half(w, u, h);
+    reconstruct<MU, SG>(acc, w);
   } else {
-    load_link<G>(u, U, MU, 1 - e, j, vh);
-    su3_adj_mul_half(w, u, h);
+    if constexpr (MU == 3) {
+      V p[6];
+      in.load_pair_raw(p, j * NR + r, SG < 0 ? 2 : 0);  // values as stored: factor 1
+      const A f = (A)2 * sgn;
+#pragma unroll
+      for (int cc = 0; cc < 3; cc++) {
+        h[2 * cc] = cscale<V>(f, p[cc]);
+        h[2 * cc + 1] = cscale<V>(f, p[3 + cc]);
+      }
+    } else {
+      V p[12];
+      in.load(p, j * NR + r);
+      project<MU, SG>(h, p);
+      if (sgn < (A)0) {
+#pragma unroll
+        for (int k = 0; k < 6; k++) h[k] = cmk<V>(-h[k].x, -h[k].y);
+      }
+    }
+    V u[9];
+    if (FWD) {
+      load_link<G>(u, U, MU, e, i, vh);
+      su3_mul_half(w, u, h);
+    } else {
+      load_link<G>(u, U, MU, 1 - e, j, vh);
+      su3_adj_mul_half(w, u, h);
+    }
+    reconstruct<MU, SG>(acc, w);
   }
-  reconstruct<MU, SG>(acc, w);
 }
 
 // Completes one hop before the next starts: the accumulator is pinned in
@@ -65,5 +92,36 @@ template <class V> __device__ __forceinline__ void pin(V (&acc)[12]) {
   for (int k = 0; k < 12; k++) asm volatile("" : "+v"(acc[k].x), "+v"(acc[k].y));
 }
 #define HB_SCHED_FENCE() do { pin(acc); __builtin_amdgcn_sched_barrier(0); } while (0)
+
+// Hops issued between fences: the loads of the hops of one group are in
+// flight together.  FP64 keeps one (its 16-byte loads already hold enough
+// bytes in flight per wave, and two FP64 hops spill); the reduced precisions
+// load 4- and 8-byte words, whose waves otherwise wait on memory latency.
+#ifndef HB_HOPS_PER_FENCE_REDUCED
+#define HB_HOPS_PER_FENCE_REDUCED 1
+#endif
+template <class S> constexpr int hops_per_fence() {
+  return sizeof(typename S::ST) == 16 ? 1 : HB_HOPS_PER_FENCE_REDUCED;
+}
+
+// all 8 hops of the hopping matrix into acc, t first (the reference's
+// direction order does not matter for the sum); the forward projector sign is
+// (1 - gamma) for D and (1 + gamma) for D^+
+template <class S, class G, bool DAG, int NR = 1>
+__device__ __forceinline__ void hop_all(typename S::V (&acc)[12], const typename G::ST *U,
+                                        const SpinorView<S> &in, const Geom &g, int e, int i,
+                                        const int (&xc)[4], int r = 0) {
+  constexpr int F = hops_per_fence<S>();
+  constexpr int SF = DAG ? 1 : -1;
+  constexpr int SB = -SF;
+#define HB_HOP(MU, FWD, SG, N)                                    \
+  hop<S, G, MU, FWD, SG, NR>(acc, U, in, g, e, i, xc, r);         \
+  if constexpr ((N) % F == 0) HB_SCHED_FENCE();
+  HB_HOP(3, true, SF, 1) HB_HOP(3, false, SB, 2)
+  HB_HOP(2, true, SF, 3) HB_HOP(2, false, SB, 4)
+  HB_HOP(1, true, SF, 5) HB_HOP(1, false, SB, 6)
+  HB_HOP(0, true, SF, 7) HB_HOP(0, false, SB, 8)
+#undef HB_HOP
+}
 
 }  // namespace hb

@@ -17,6 +17,7 @@
 #include <hip/hip_fp16.h>
 
 #include <cstdint>
+#include <type_traits>
 #include <cstdio>
 #include <string>
 #include <vector>
@@ -167,15 +168,31 @@ template <typename V> __device__ __forceinline__ V cmi(V a) { return cmk<V>(a.y,
 template <typename V, typename S> __device__ __forceinline__ V cscale(S s, V a) {
   return cmk<V>(s * a.x, s * a.y);
 }
-// acc += a * b
+// acc += a * b.  FP32 (the reduced-precision kernels, which are bound by
+// VALU issue) chains four FMAs into acc; FP64 keeps acc + (product), the
+// rounding the FP64 operator has always had (it is bound by memory).
 template <typename V> __device__ __forceinline__ void cfma(V &acc, V a, V b) {
-  acc.x += a.x * b.x - a.y * b.y;
-  acc.y += a.x * b.y + a.y * b.x;
+  if constexpr (std::is_same<decltype(V{}.x), float>::value) {
+    acc.x = fmaf(a.x, b.x, acc.x);
+    acc.x = fmaf(-a.y, b.y, acc.x);
+    acc.y = fmaf(a.x, b.y, acc.y);
+    acc.y = fmaf(a.y, b.x, acc.y);
+  } else {
+    acc.x += a.x * b.x - a.y * b.y;
+    acc.y += a.x * b.y + a.y * b.x;
+  }
 }
 // acc += conj(a) * b
 template <typename V> __device__ __forceinline__ void cfmac(V &acc, V a, V b) {
-  acc.x += a.x * b.x + a.y * b.y;
-  acc.y += a.x * b.y - a.y * b.x;
+  if constexpr (std::is_same<decltype(V{}.x), float>::value) {
+    acc.x = fmaf(a.x, b.x, acc.x);
+    acc.x = fmaf(a.y, b.y, acc.x);
+    acc.y = fmaf(a.x, b.y, acc.y);
+    acc.y = fmaf(-a.y, b.x, acc.y);
+  } else {
+    acc.x += a.x * b.x + a.y * b.y;
+    acc.y += a.x * b.y - a.y * b.x;
+  }
 }
 
 // ----------------------------------------------------------------------------

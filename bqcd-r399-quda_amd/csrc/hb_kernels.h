@@ -4,6 +4,8 @@
 // and the packed 6x6 clover block products.
 #pragma once
 
+#include <type_traits>
+
 #include "hb_internal.h"
 
 namespace hb {
@@ -87,10 +89,18 @@ struct SF16 {
   static constexpr float kScale = 32767.0f;
   static constexpr float kInv = 1.0f / 32767.0f;
   __device__ __forceinline__ static V cvt(ST s) { return make_float2(s.x * kInv, s.y * kInv); }
+  // the stored integers as floats (value * 32767): kernels fold the 1/32767
+  // into one per-site or per-hop scale instead of one multiply per word
+  __device__ __forceinline__ static V raw(ST s) { return make_float2((float)s.x, (float)s.y); }
   __device__ __forceinline__ static short q(float v) {
     float r = rintf(v * kScale);
     r = fminf(fmaxf(r, -kScale), kScale);
     return (short)r;
+  }
+  // the pair (x, y) in [-1, 1] to 16 bits in one instruction (round to nearest,
+  // clamped to +-32767, as q)
+  __device__ __forceinline__ static ST qpack(float x, float y) {
+    return __builtin_bit_cast(ST, __builtin_amdgcn_cvt_pknorm_i16(x, y));
   }
   __device__ __forceinline__ static V ld(const ST *p, size_t k) { return cvt(p[k]); }
   __device__ __forceinline__ static void st(ST *p, size_t k, V v) { p[k] = make_short2(q(v.x), q(v.y)); }
@@ -101,6 +111,10 @@ struct SF16 {
   template <int POL = 0>
   __device__ __forceinline__ static V bld(rsrc_t r, uint32_t vo, uint32_t so) {
     return cvt(__builtin_bit_cast(ST, __builtin_amdgcn_raw_buffer_load_b32(r, vo, so, POL)));
+  }
+  template <int POL = 0>
+  __device__ __forceinline__ static V bld_raw(rsrc_t r, uint32_t vo, uint32_t so) {
+    return raw(__builtin_bit_cast(ST, __builtin_amdgcn_raw_buffer_load_b32(r, vo, so, POL)));
   }
 };
 
@@ -119,25 +133,41 @@ template <class S> struct SpinorView {
     r = make_rsrc(f.p, plane * 12u);
     if constexpr (S::kNorm) rn = make_rsrc(f.norm, (uint32_t)vh * 4u);
   }
-  __device__ __forceinline__ void load(typename S::V (&v)[12], int i) const {
+  // the spinor of site i as stored: for 16-bit storage the raw integers, with
+  // the returned factor (site norm / 32767) still to be applied -- the hopping
+  // term applies it once to the projected half spinor (12 instead of 48
+  // multiplies); other storages return the values and 1
+  __device__ __forceinline__ typename S::A load_raw(typename S::V (&v)[12], int i) const {
     const uint32_t vo = (uint32_t)i * S::kBytes;
-#pragma unroll
-    for (int k = 0; k < 12; k++) v[k] = S::bld(r, vo, plane * k);
     if constexpr (S::kNorm) {
-      float n = bld_f32(rn, (uint32_t)i * 4u, 0);
 #pragma unroll
-      for (int k = 0; k < 12; k++) { v[k].x *= n; v[k].y *= n; }
+      for (int k = 0; k < 12; k++) v[k] = S::bld_raw(r, vo, plane * k);
+      return bld_f32(rn, (uint32_t)i * 4u, 0) * S::kInv;
+    } else {
+#pragma unroll
+      for (int k = 0; k < 12; k++) v[k] = S::bld(r, vo, plane * k);
+      return (typename S::A)1;
     }
   }
-  // spin components s0, s0+1 (k = 3*s0 .. 3*s0+5) -- the t projector
-  __device__ __forceinline__ void load_pair(typename S::V (&v)[6], int i, int s0) const {
+  // spin components s0, s0+1 (k = 3*s0 .. 3*s0+5) -- the t projector; same
+  // convention as load_raw
+  __device__ __forceinline__ typename S::A load_pair_raw(typename S::V (&v)[6], int i, int s0) const {
     const uint32_t vo = (uint32_t)i * S::kBytes;
-#pragma unroll
-    for (int k = 0; k < 6; k++) v[k] = S::bld(r, vo, plane * (3 * s0 + k));
     if constexpr (S::kNorm) {
-      float n = bld_f32(rn, (uint32_t)i * 4u, 0);
 #pragma unroll
-      for (int k = 0; k < 6; k++) { v[k].x *= n; v[k].y *= n; }
+      for (int k = 0; k < 6; k++) v[k] = S::bld_raw(r, vo, plane * (3 * s0 + k));
+      return bld_f32(rn, (uint32_t)i * 4u, 0) * S::kInv;
+    } else {
+#pragma unroll
+      for (int k = 0; k < 6; k++) v[k] = S::bld(r, vo, plane * (3 * s0 + k));
+      return (typename S::A)1;
+    }
+  }
+  __device__ __forceinline__ void load(typename S::V (&v)[12], int i) const {
+    const typename S::A f = load_raw(v, i);
+    if constexpr (S::kNorm) {
+#pragma unroll
+      for (int k = 0; k < 12; k++) { v[k].x *= f; v[k].y *= f; }
     }
   }
 };
@@ -145,12 +175,17 @@ template <class S> struct SpinorView {
 template <class S>
 __device__ __forceinline__ void load_spinor(typename S::V (&v)[12], const SpinorPtr<S> &f, int i,
                                             int vh) {
-#pragma unroll
-  for (int k = 0; k < 12; k++) v[k] = S::ld(f.p, (size_t)k * vh + i);
   if constexpr (S::kNorm) {
-    float n = f.norm[i];
+    const float n = f.norm[i] * S::kInv;
 #pragma unroll
-    for (int k = 0; k < 12; k++) { v[k].x *= n; v[k].y *= n; }
+    for (int k = 0; k < 12; k++) {
+      v[k] = S::raw(f.p[(size_t)k * vh + i]);
+      v[k].x *= n;
+      v[k].y *= n;
+    }
+  } else {
+#pragma unroll
+    for (int k = 0; k < 12; k++) v[k] = S::ld(f.p, (size_t)k * vh + i);
   }
 }
 
@@ -171,8 +206,7 @@ __device__ __forceinline__ void store_spinor(const SpinorPtr<S> &f, int i, int v
     float inv = m > 0.f ? 1.0f / m : 0.f;
     f.norm[i] = m;
 #pragma unroll
-    for (int k = 0; k < 12; k++)
-      S::st(f.p, (size_t)k * vh + i, cmk<V>((A)v[k].x * inv, (A)v[k].y * inv));
+    for (int k = 0; k < 12; k++) f.p[(size_t)k * vh + i] = S::qpack((float)v[k].x * inv, (float)v[k].y * inv);
   } else if constexpr (sizeof(typename S::ST) == 16) {
 #pragma unroll
     for (int k = 0; k < 12; k++) S::st_nt(f.p, (size_t)k * vh + i, cmk<V>((A)v[k].x, (A)v[k].y));
@@ -209,6 +243,38 @@ __device__ __forceinline__ void load_link(typename S::V (&u)[9], const typename 
     u[6] = cross(u[1], u[5], u[2], u[4]);
     u[7] = cross(u[2], u[3], u[0], u[5]);
     u[8] = cross(u[0], u[4], u[1], u[3]);
+  }
+}
+
+// 16-bit links are used as stored (the integers as floats, value * 32767):
+// load_link_f returns the factor 1/32767 that the caller folds into the half
+// spinor, and rebuilds row 3 = conj(row0 x row1) / 32767 in the same units.
+// Other storages load the link as load_link and return 1.
+template <class G> constexpr bool link_raw() { return std::is_same<G, SF16>::value; }
+
+template <class G, int POL = kNT>
+__device__ __forceinline__ typename G::A load_link_f(typename G::V (&u)[9], const typename G::ST *U,
+                                                     int mu, int eo, int i, int vh) {
+  if constexpr (link_raw<G>()) {
+    using V = typename G::V;
+    constexpr int NP = link_planes<G>();
+    const uint32_t plane = (uint32_t)vh * G::kBytes;
+    rsrc_t r = make_rsrc(U + (size_t)((mu * 2 + eo) * NP) * vh, plane * (uint32_t)NP);
+    const uint32_t vo = (uint32_t)i * G::kBytes;
+#pragma unroll
+    for (int k = 0; k < NP; k++) u[k] = G::template bld_raw<POL>(r, vo, plane * k);
+    auto cross = [](V a0, V b1, V a1, V b0) {  // conj(a0 b1 - a1 b0) / 32767
+      V t = cmul(a0, b1);
+      V s = cmul(a1, b0);
+      return cmk<V>((t.x - s.x) * G::kInv, -(t.y - s.y) * G::kInv);
+    };
+    u[6] = cross(u[1], u[5], u[2], u[4]);
+    u[7] = cross(u[2], u[3], u[0], u[5]);
+    u[8] = cross(u[0], u[4], u[1], u[3]);
+    return G::kInv;
+  } else {
+    load_link<G, POL>(u, U, mu, eo, i, vh);
+    return (typename G::A)1;
   }
 }
 
@@ -414,6 +480,30 @@ __device__ __forceinline__ void load_clover_block(typename S::V (&w)[18], const 
   }
 }
 
+// block J as stored in 16 bits (the integers Q as floats) and its factors:
+// the block is c 1 + (m / 32767) Q_int; returns (m / 32767, c)
+__device__ __forceinline__ float2 load_clover_block_raw(float2 (&w)[18], const SF16::ST *C,
+                                                        const float *Cn, int eo, int J, int i,
+                                                        int vh) {
+  const uint32_t plane = (uint32_t)vh * SF16::kBytes;
+  rsrc_t r = make_rsrc(C + (size_t)(eo * 36 + J * 18) * vh, plane * 18u);
+  const uint32_t vo = (uint32_t)i * SF16::kBytes;
+#pragma unroll
+  for (int k = 0; k < 18; k++) w[k] = SF16::bld_raw<kNT>(r, vo, plane * k);
+  const float2 mc = reinterpret_cast<const float2 *>(Cn)[(size_t)(eo * 2 + J) * vh + i];
+  return make_float2(mc.x * SF16::kInv, mc.y);
+}
+
+// y = (c 1 + f Q) x for one block from y = Q x (mc = (f, c))
+template <class V>
+__device__ __forceinline__ void clover_raw_finish(V (&y)[6], const V (&x)[6], float2 mc) {
+#pragma unroll
+  for (int m = 0; m < 6; m++) {
+    y[m].x = y[m].x * mc.x + x[m].x * mc.y;
+    y[m].y = y[m].y * mc.x + x[m].y * mc.y;
+  }
+}
+
 // out = A in (Dirac basis) with A stored as two chiral blocks of A/2
 // (clover_mult_a.F90:44-95); mode 0 = Hermitian product, 1 = LDL^+ solve.
 // SG is the clover storage policy, V the arithmetic complex type.
@@ -422,15 +512,20 @@ __device__ __forceinline__ void clover_apply(V (&out)[12], const V (&in)[12],
                                              const typename SG::ST *C, const float *Cn, int eo,
                                              int i, int vh) {
   V xp[6], y1[6], y2[6];
+  constexpr bool kRaw = std::is_same<SG, SF16>::value;
+  static_assert(!kRaw || MODE == 0, "16-bit clover blocks hold A/2 or A^-1/2, not LDL factors");
 #pragma unroll
   for (int m = 0; m < 6; m++) xp[m] = cadd(in[m], in[m + 6]);
   {
     typename SG::V w[18];
-    load_clover_block<SG>(w, C, Cn, eo, 0, i, vh);
+    float2 mc;
+    if constexpr (kRaw) mc = load_clover_block_raw(w, C, Cn, eo, 0, i, vh);
+    else load_clover_block<SG>(w, C, Cn, eo, 0, i, vh);
     V ww[18];
 #pragma unroll
     for (int k = 0; k < 18; k++) ww[k] = cmk<V>(w[k].x, w[k].y);
     if constexpr (MODE == 0) herm6(y1, ww, xp); else ldl6(y1, ww, xp);
+    if constexpr (kRaw) clover_raw_finish(y1, xp, mc);
   }
   // finish block 0 before loading block 1 (bounds live registers)
 #pragma unroll
@@ -440,11 +535,14 @@ __device__ __forceinline__ void clover_apply(V (&out)[12], const V (&in)[12],
   for (int m = 0; m < 6; m++) xp[m] = csub(in[m], in[m + 6]);
   {
     typename SG::V w[18];
-    load_clover_block<SG>(w, C, Cn, eo, 1, i, vh);
+    float2 mc;
+    if constexpr (kRaw) mc = load_clover_block_raw(w, C, Cn, eo, 1, i, vh);
+    else load_clover_block<SG>(w, C, Cn, eo, 1, i, vh);
     V ww[18];
 #pragma unroll
     for (int k = 0; k < 18; k++) ww[k] = cmk<V>(w[k].x, w[k].y);
     if constexpr (MODE == 0) herm6(y2, ww, xp); else ldl6(y2, ww, xp);
+    if constexpr (kRaw) clover_raw_finish(y2, xp, mc);
   }
 #pragma unroll
   for (int m = 0; m < 6; m++) {

@@ -72,20 +72,39 @@ __device__ __forceinline__ void load_clover_block_g(typename SG::V (&w)[18], con
   }
 }
 
+// the same block as stored in 16 bits (integers as floats); returns
+// (m / 32767, c) -- the block is c 1 + (m / 32767) Q_int
+__device__ __forceinline__ float2 load_clover_block_g_raw(float2 (&w)[18], const SF16::ST *C,
+                                                          const float *Cn, int ngroups, int gidx,
+                                                          int eo, int J, int i, int vh) {
+  const uint32_t plane = (uint32_t)vh * SF16::kBytes;
+  rsrc_t r = make_rsrc(C, plane * 72u * (uint32_t)ngroups);
+  const uint32_t vo = ((uint32_t)gidx * 72u * (uint32_t)vh + (uint32_t)i) * SF16::kBytes;
+  const uint32_t base = (uint32_t)(eo * 36 + J * 18);
+#pragma unroll
+  for (int k = 0; k < 18; k++) w[k] = SF16::bld_raw<kNT>(r, vo, plane * (base + k));
+  const float2 mc = reinterpret_cast<const float2 *>(Cn)[((size_t)gidx * 4 + eo * 2 + J) * vh + i];
+  return make_float2(mc.x * SF16::kInv, mc.y);
+}
+
 template <class SG, class V>
 __device__ __forceinline__ void clover_apply_g(V (&out)[12], const V (&in)[12], const typename SG::ST *C,
                                                const float *Cn, int ngroups, int gidx, int eo, int i,
                                                int vh) {
+  constexpr bool kRaw = std::is_same<SG, SF16>::value;
   V xp[6], y1[6], y2[6];
 #pragma unroll
   for (int m = 0; m < 6; m++) xp[m] = cadd(in[m], in[m + 6]);
   {
     typename SG::V w[18];
-    load_clover_block_g<SG>(w, C, Cn, ngroups, gidx, eo, 0, i, vh);
+    float2 mc;
+    if constexpr (kRaw) mc = load_clover_block_g_raw(w, C, Cn, ngroups, gidx, eo, 0, i, vh);
+    else load_clover_block_g<SG>(w, C, Cn, ngroups, gidx, eo, 0, i, vh);
     V ww[18];
 #pragma unroll
     for (int k = 0; k < 18; k++) ww[k] = cmk<V>(w[k].x, w[k].y);
     herm6(y1, ww, xp);
+    if constexpr (kRaw) clover_raw_finish(y1, xp, mc);
   }
 #pragma unroll
   for (int m = 0; m < 6; m++) asm volatile("" : "+v"(y1[m].x), "+v"(y1[m].y));
@@ -94,11 +113,14 @@ __device__ __forceinline__ void clover_apply_g(V (&out)[12], const V (&in)[12], 
   for (int m = 0; m < 6; m++) xp[m] = csub(in[m], in[m + 6]);
   {
     typename SG::V w[18];
-    load_clover_block_g<SG>(w, C, Cn, ngroups, gidx, eo, 1, i, vh);
+    float2 mc;
+    if constexpr (kRaw) mc = load_clover_block_g_raw(w, C, Cn, ngroups, gidx, eo, 1, i, vh);
+    else load_clover_block_g<SG>(w, C, Cn, ngroups, gidx, eo, 1, i, vh);
     V ww[18];
 #pragma unroll
     for (int k = 0; k < 18; k++) ww[k] = cmk<V>(w[k].x, w[k].y);
     herm6(y2, ww, xp);
+    if constexpr (kRaw) clover_raw_finish(y2, xp, mc);
   }
 #pragma unroll
   for (int m = 0; m < 6; m++) {
@@ -124,16 +146,7 @@ __global__ __launch_bounds__(kBlock) void k_dslash_multi(MultiArgs<S, G> a) {
     int xc[4];
     site_coords(g, e, site, xc[0], xc[1], xc[2], xc[3]);
     const SpinorView<S> in(a.in, vhe);
-    constexpr int SF = DAG ? 1 : -1;
-    constexpr int SB = -SF;
-    hop<S, G, 3, true, SF, NR>(acc, a.U, in, g, e, site, xc, r);  HB_SCHED_FENCE();
-    hop<S, G, 3, false, SB, NR>(acc, a.U, in, g, e, site, xc, r); HB_SCHED_FENCE();
-    hop<S, G, 2, true, SF, NR>(acc, a.U, in, g, e, site, xc, r);  HB_SCHED_FENCE();
-    hop<S, G, 2, false, SB, NR>(acc, a.U, in, g, e, site, xc, r); HB_SCHED_FENCE();
-    hop<S, G, 1, true, SF, NR>(acc, a.U, in, g, e, site, xc, r);  HB_SCHED_FENCE();
-    hop<S, G, 1, false, SB, NR>(acc, a.U, in, g, e, site, xc, r); HB_SCHED_FENCE();
-    hop<S, G, 0, true, SF, NR>(acc, a.U, in, g, e, site, xc, r);  HB_SCHED_FENCE();
-    hop<S, G, 0, false, SB, NR>(acc, a.U, in, g, e, site, xc, r); HB_SCHED_FENCE();
+    hop_all<S, G, DAG, NR>(acc, a.U, in, g, e, site, xc, r);
   }
   const int ie = site * NR + r;
   if constexpr (EPI == EPI_CLINV) {
