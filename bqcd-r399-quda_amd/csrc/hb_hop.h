@@ -96,9 +96,11 @@ template <class V> __device__ __forceinline__ void pin(V (&acc)[12]) {
 // Hops issued between fences: the loads of the hops of one group are in
 // flight together.  FP64 keeps one (its 16-byte loads already hold enough
 // bytes in flight per wave, and two FP64 hops spill); the reduced precisions
-// load 4- and 8-byte words, whose waves otherwise wait on memory latency.
+// issue two (the +mu and -mu hops), which overlaps one hop's arithmetic with
+// the other's loads (2% on the 16-bit solver; four or eight hops gain no more
+// and eight costs occupancy).
 #ifndef HB_HOPS_PER_FENCE_REDUCED
-#define HB_HOPS_PER_FENCE_REDUCED 1
+#define HB_HOPS_PER_FENCE_REDUCED 2
 #endif
 template <class S> constexpr int hops_per_fence() {
   return sizeof(typename S::ST) == 16 ? 1 : HB_HOPS_PER_FENCE_REDUCED;
