@@ -164,7 +164,7 @@ __constant__ double2 c_ts[6][4][4] = {
 };
 
 // W[6][9][vh]: w_k = transpose(conj(b) S_k a) at the sites of one parity (clover_bsa)
-__global__ void k_w_bsa(double2 *__restrict__ W, const double2 *__restrict__ b, const double2 *__restrict__ a,
+__global__ __launch_bounds__(kBlock) void k_w_bsa(double2 *__restrict__ W, const double2 *__restrict__ b, const double2 *__restrict__ a,
                         int vh) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= vh) return;
@@ -227,58 +227,132 @@ __device__ __forceinline__ M3 wmat(const double2 *__restrict__ W, int k, int i, 
   for (int m = 0; m < 9; m++) r.m[m] = W[((size_t)k * 9 + m) * vh + i];
   return r;
 }
-
-// clover_d_same_eo (clover/clover_d.F90:203-233)
-__device__ void d_same(double (&q)[8], double s, const M3 &u0, const M3 &u1, const M3 &u2, const M3 &u3,
-                       const M3 &u4, const M3 &u5, const M3 &u6, const M3 &wa, const M3 &wb, const M3 &wc) {
-  M3 u = m3_mul(u0, u1), v = m3_mul(u3, u2);
-  re_tr_j(q, m3_mul(m3_muld(u, v), wa), s);
-  re_tr_j(q, m3_muld(m3_mul(wa, v), u), s);
-  re_tr_j(q, m3_muld(m3_mul(u, wb), v), s);
-  re_tr_j(q, m3_muld(m3_mul(v, wb), u), s);
-  u = m3_muld(u0, u6);
-  v = m3_dmul(u4, u5);
-  re_tr_j(q, m3_muld(m3_mul(wa, v), u), -s);
-  re_tr_j(q, m3_mul(m3_muld(u, v), wa), -s);
-  re_tr_j(q, m3_muld(m3_mul(v, wc), u), -s);
-  re_tr_j(q, m3_muld(m3_mul(u, wc), v), -s);
-}
-
-// clover_d_diff_eo (clover/clover_d.F90:236-266)
-__device__ void d_diff(double (&q)[8], double s, const M3 &u0, const M3 &u1, const M3 &u2, const M3 &u3,
-                       const M3 &u4, const M3 &u5, const M3 &u6, const M3 &wd, const M3 &we, const M3 &wf) {
-  M3 u = m3_muld(m3_muld(u1, u2), u3);  // u1 u2^+ u3^+
-  M3 v = m3_muld(m3_muld(u2, u1), u0);  // u2 u1^+ u0^+
-  re_tr_j(q, m3_mul(m3_mul(u0, wd), u), s);
-  re_tr_j(q, m3_muld(m3_dmul(u, wd), u0), s);
-  re_tr_j(q, m3_muld(m3_dmul(v, we), u3), s);
-  re_tr_j(q, m3_mul(m3_mul(u3, we), v), s);
-  u = m3_mul(m3_dag(m3_mul(u5, u6)), u4);  // u6^+ u5^+ u4
-  v = m3_muld(m3_muld(u0, u6), u5);        // u0 u6^+ u5^+
-  re_tr_j(q, m3_muld(m3_dmul(u, wd), u0), -s);
-  re_tr_j(q, m3_mul(m3_mul(u0, wd), u), -s);
-  re_tr_j(q, m3_muld(m3_dmul(u4, wf), v), -s);
-  re_tr_j(q, m3_mul(m3_mul(v, wf), u4), -s);
+// the same through a buffer resource over matrix k's nine planes (k uniform)
+__device__ __forceinline__ M3 wmat_b(const double2 *__restrict__ W, int k, int i, int vh) {
+  const uint32_t plane = (uint32_t)vh * 16u;
+  const rsrc_t r = make_rsrc(W + (size_t)k * 9 * vh, plane * 9u);
+  const uint32_t vo = (uint32_t)i * 16u;
+  M3 m;
+#pragma unroll
+  for (int q = 0; q < 9; q++) m.m[q] = bld_f64x2(r, vo, plane * (uint32_t)q);
+  return m;
 }
 
 // clover_d_loop (clover/clover_d.F90:152-200) for one link (mu, e, i) and
-// plane partner nu; the w fields live on parity weo
+// plane partner nu; the w fields live on parity weo.  The eight terms of
+// clover_d_same_eo (:203-233) or clover_d_diff_eo (:236-266) are evaluated in
+// groups that load their links and w matrices when needed (reloads hit the
+// caches), with a scheduling barrier between groups, so that at most five
+// 3x3 matrices are live: loading all seven links and three w matrices first
+// needs ~360 registers and spills.
+#define HB_GROUP_END() __builtin_amdgcn_sched_barrier(0)
+// an index the compiler cannot prove equal to its argument: a matrix loaded
+// again through it is a fresh load, not a register kept live since the first
+__device__ __forceinline__ int fresh(int i) {
+  asm volatile("" : "+v"(i));
+  return i;
+}
 __device__ void d_loop(double (&q)[8], const double2 *__restrict__ U, const Geom &g, int e, int mu, int nu,
                        int i, const int (&x)[4], const double2 *__restrict__ W, int k, int weo, double s) {
   const int o = 1 - e, vh = g.vh;
   const int id = nb(g, x[0], x[1], x[2], x[3], mu, 1);
   const int ie = nb(g, x[0], x[1], x[2], x[3], nu, 1);
   const int iff = nb(g, x[0], x[1], x[2], x[3], nu, -1);
-  const int ib = nb2(g, x[0], x[1], x[2], x[3], mu, 1, nu, 1);
-  const int ic = nb2(g, x[0], x[1], x[2], x[3], mu, 1, nu, -1);
-  const M3 u0 = link(U, mu, e, i, vh), u1 = link(U, nu, o, id, vh), u2 = link(U, mu, o, ie, vh);
-  const M3 u3 = link(U, nu, e, i, vh), u4 = link(U, nu, o, iff, vh), u5 = link(U, mu, o, iff, vh);
-  const M3 u6 = link(U, nu, e, ic, vh);
-  if (e == weo)
-    d_same(q, s, u0, u1, u2, u3, u4, u5, u6, wmat(W, k, i, vh), wmat(W, k, ib, vh), wmat(W, k, ic, vh));
-  else
-    d_diff(q, s, u0, u1, u2, u3, u4, u5, u6, wmat(W, k, id, vh), wmat(W, k, ie, vh), wmat(W, k, iff, vh));
+  if (e == weo) {
+    // clover_d_same_eo
+    const int ib = nb2(g, x[0], x[1], x[2], x[3], mu, 1, nu, 1);
+    const int ic = nb2(g, x[0], x[1], x[2], x[3], mu, 1, nu, -1);
+    {
+      const M3 u = m3_mul(link_b(U, mu, e, i, vh), link_b(U, nu, o, id, vh));   // u0 u1
+      HB_GROUP_END();
+      const M3 v = m3_mul(link_b(U, nu, e, i, vh), link_b(U, mu, o, ie, vh));   // u3 u2
+      HB_GROUP_END();
+      {
+        const M3 wa = wmat_b(W, k, i, vh);
+        re_tr_j_prod<0>(q, m3_muld(u, v), wa, s);
+        HB_GROUP_END();
+        re_tr_j_prod<2>(q, m3_mul(wa, v), u, s);
+      }
+      HB_GROUP_END();
+      {
+        const M3 wb = wmat_b(W, k, ib, vh);
+        re_tr_j_prod<2>(q, m3_mul(u, wb), v, s);
+        HB_GROUP_END();
+        re_tr_j_prod<2>(q, m3_mul(v, wb), u, s);
+      }
+    }
+    HB_GROUP_END();
+    {
+      const M3 u = m3_muld(link_b(U, mu, e, fresh(i), vh), link_b(U, nu, e, ic, vh));  // u0 u6^+
+      HB_GROUP_END();
+      const M3 v = m3_dmul(link_b(U, nu, o, iff, vh), link_b(U, mu, o, iff, vh));  // u4^+ u5
+      HB_GROUP_END();
+      {
+        const M3 wa = wmat_b(W, k, fresh(i), vh);
+        re_tr_j_prod<2>(q, m3_mul(wa, v), u, -s);
+        HB_GROUP_END();
+        re_tr_j_prod<0>(q, m3_muld(u, v), wa, -s);
+      }
+      HB_GROUP_END();
+      {
+        const M3 wc = wmat_b(W, k, ic, vh);
+        re_tr_j_prod<2>(q, m3_mul(v, wc), u, -s);
+        HB_GROUP_END();
+        re_tr_j_prod<2>(q, m3_mul(u, wc), v, -s);
+      }
+    }
+  } else {
+    // clover_d_diff_eo
+    {
+      const M3 t = m3_muld(link_b(U, nu, o, id, vh), link_b(U, mu, o, ie, vh));  // u1 u2^+
+      HB_GROUP_END();
+      const M3 u = m3_muld(t, link_b(U, nu, e, i, vh));                          // u1 u2^+ u3^+
+      HB_GROUP_END();
+      {
+        const M3 wd = wmat_b(W, k, id, vh);
+        const M3 u0 = link_b(U, mu, e, fresh(i), vh);
+        re_tr_j_prod<0>(q, m3_mul(u0, wd), u, s);
+        HB_GROUP_END();
+        re_tr_j_prod<2>(q, m3_dmul(u, wd), u0, s);
+      }
+      HB_GROUP_END();
+      const M3 v = m3_dmul(t, m3_dag(link_b(U, mu, e, fresh(i), vh)));                  // u2 u1^+ u0^+
+      HB_GROUP_END();
+      {
+        const M3 we = wmat_b(W, k, ie, vh);
+        const M3 u3 = link_b(U, nu, e, fresh(i), vh);
+        re_tr_j_prod<2>(q, m3_dmul(v, we), u3, s);
+        HB_GROUP_END();
+        re_tr_j_prod<0>(q, m3_mul(u3, we), v, s);
+      }
+    }
+    HB_GROUP_END();
+    {
+      const M3 t = m3_mul(link_b(U, mu, o, iff, vh), link_b(U, nu, e, nb2(g, x[0], x[1], x[2], x[3], mu, 1, nu, -1), vh));  // u5 u6
+      HB_GROUP_END();
+      const M3 u = m3_dmul(t, link_b(U, nu, o, iff, vh));                        // u6^+ u5^+ u4
+      HB_GROUP_END();
+      {
+        const M3 wd = wmat_b(W, k, fresh(id), vh);
+        const M3 u0 = link_b(U, mu, e, fresh(i), vh);
+        re_tr_j_prod<2>(q, m3_dmul(u, wd), u0, -s);
+        HB_GROUP_END();
+        re_tr_j_prod<0>(q, m3_mul(u0, wd), u, -s);
+      }
+      HB_GROUP_END();
+      const M3 v = m3_muld(link_b(U, mu, e, fresh(i), vh), t);                          // u0 u6^+ u5^+
+      HB_GROUP_END();
+      {
+        const M3 wf = wmat_b(W, k, iff, vh);
+        const M3 u4 = link_b(U, nu, o, fresh(iff), vh);
+        re_tr_j_prod<2>(q, m3_dmul(u4, wf), v, -s);
+        HB_GROUP_END();
+        re_tr_j_prod<0>(q, m3_mul(v, wf), u4, -s);
+      }
+    }
+  }
 }
+#undef HB_GROUP_END
 
 // all clover_d contributions of one link: up to two (W, parity, scale) sets
 __global__ __launch_bounds__(64) void k_clover_force(double *P, const double2 *__restrict__ U, Geom g,
@@ -290,11 +364,15 @@ __global__ __launch_bounds__(64) void k_clover_force(double *P, const double2 *_
   int x[4];
   site_coords(g, e, i, x[0], x[1], x[2], x[3]);
   double q[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  // kept as loops: unrolled, the compiler merges the link loads of all twelve
+  // (set, plane) bodies and keeps them live across the kernel
+#pragma unroll 1
   for (int set = 0; set < 2; set++) {
     const double2 *W = set == 0 ? W0 : W1;
     const double s = set == 0 ? s0 : s1;
     const int weo = set == 0 ? weo0 : weo1;
     if (!W || s == 0.0) continue;
+#pragma unroll 1
     for (int k = 0; k < 6; k++) {
       const int a = c_pairs[k][0], b = c_pairs[k][1];
       if (a != mu && b != mu) continue;
@@ -322,11 +400,11 @@ __global__ __launch_bounds__(128) void k_gauge_force(double *P, const double2 *_
     const int j3 = nb(g, x[0], x[1], x[2], x[3], nu, -1);
     const int j4 = nb2(g, x[0], x[1], x[2], x[3], nu, -1, mu, 1);
     // U_nu(x+mu) U_mu(x+nu)^+ U_nu(x)^+  +  U_nu(x+mu-nu)^+ U_mu(x-nu)^+ U_nu(x-nu)
-    m3_acc(st, m3_muld(m3_muld(link(U, nu, o, j1, vh), link(U, mu, o, j2, vh)), link(U, nu, e, i, vh)));
-    m3_acc(st, m3_mul(m3_dag(m3_mul(link(U, mu, o, j3, vh), link(U, nu, e, j4, vh))), link(U, nu, o, j3, vh)));
+    m3_acc(st, m3_muld(m3_muld(link_b(U, nu, o, j1, vh), link_b(U, mu, o, j2, vh)), link_b(U, nu, e, i, vh)));
+    m3_acc(st, m3_mul(m3_dag(m3_mul(link_b(U, mu, o, j3, vh), link_b(U, nu, e, j4, vh))), link_b(U, nu, o, j3, vh)));
   }
   double q[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-  im_tr_j(q, m3_mul(link(U, mu, e, i, vh), st), s);
+  im_tr_j(q, m3_mul(link_b(U, mu, e, i, vh), st), s);
   mom_add(P, mu, e, i, vh, q);
 }
 

@@ -29,47 +29,60 @@ __device__ __forceinline__ M3 m3_dag(const M3 &a) {
     for (int j = 0; j < 3; j++) r.m[3 * i + j] = cconj(a.m[3 * j + i]);
   return r;
 }
-__device__ __forceinline__ M3 m3_mul(const M3 &a, const M3 &b) {
+// s += a' b' with a' = conj(a) if CA, b' = conj(b) if CB: four chained FMAs
+// (the force, clover and link-update kernels are bound by FP64 issue; the
+// sign flips are free operand modifiers)
+template <bool CA, bool CB>
+__device__ __forceinline__ void cfma3(double2 &s, double2 a, double2 b) {
+  const double ay = CA ? -a.y : a.y, by = CB ? -b.y : b.y;
+  s.x = fma(a.x, b.x, s.x);
+  s.x = fma(-ay, by, s.x);
+  s.y = fma(a.x, by, s.y);
+  s.y = fma(ay, b.x, s.y);
+}
+// Re(a' b') only
+template <bool CA, bool CB>
+__device__ __forceinline__ double cfma3_re(double s, double2 a, double2 b) {
+  const double ay = CA ? -a.y : a.y, by = CB ? -b.y : b.y;
+  return fma(-ay, by, fma(a.x, b.x, s));
+}
+
+// element (i, j) of a b (MODE 0), a^+ b (MODE 1) or a b^+ (MODE 2)
+template <int MODE>
+__device__ __forceinline__ double2 m3_elem(const M3 &a, const M3 &b, int i, int j) {
+  double2 s = make_double2(0.0, 0.0);
+#pragma unroll
+  for (int k = 0; k < 3; k++) {
+    if constexpr (MODE == 0) cfma3<false, false>(s, a.m[3 * i + k], b.m[3 * k + j]);
+    else if constexpr (MODE == 1) cfma3<true, false>(s, a.m[3 * k + i], b.m[3 * k + j]);
+    else cfma3<false, true>(s, a.m[3 * i + k], b.m[3 * j + k]);
+  }
+  return s;
+}
+template <int MODE>
+__device__ __forceinline__ double m3_elem_re(const M3 &a, const M3 &b, int i, int j) {
+  double s = 0.0;
+#pragma unroll
+  for (int k = 0; k < 3; k++) {
+    if constexpr (MODE == 0) s = cfma3_re<false, false>(s, a.m[3 * i + k], b.m[3 * k + j]);
+    else if constexpr (MODE == 1) s = cfma3_re<true, false>(s, a.m[3 * k + i], b.m[3 * k + j]);
+    else s = cfma3_re<false, true>(s, a.m[3 * i + k], b.m[3 * j + k]);
+  }
+  return s;
+}
+template <int MODE> __device__ __forceinline__ M3 m3_prod(const M3 &a, const M3 &b) {
   M3 r;
 #pragma unroll
   for (int i = 0; i < 3; i++)
 #pragma unroll
-    for (int j = 0; j < 3; j++) {
-      double2 s = cmul(a.m[3 * i], b.m[j]);
-      cfma(s, a.m[3 * i + 1], b.m[3 + j]);
-      cfma(s, a.m[3 * i + 2], b.m[6 + j]);
-      r.m[3 * i + j] = s;
-    }
+    for (int j = 0; j < 3; j++) r.m[3 * i + j] = m3_elem<MODE>(a, b, i, j);
   return r;
 }
+__device__ __forceinline__ M3 m3_mul(const M3 &a, const M3 &b) { return m3_prod<0>(a, b); }
 // a^+ b
-__device__ __forceinline__ M3 m3_dmul(const M3 &a, const M3 &b) {
-  M3 r;
-#pragma unroll
-  for (int i = 0; i < 3; i++)
-#pragma unroll
-    for (int j = 0; j < 3; j++) {
-      double2 s = cmulc(a.m[i], b.m[j]);
-      cfmac(s, a.m[3 + i], b.m[3 + j]);
-      cfmac(s, a.m[6 + i], b.m[6 + j]);
-      r.m[3 * i + j] = s;
-    }
-  return r;
-}
+__device__ __forceinline__ M3 m3_dmul(const M3 &a, const M3 &b) { return m3_prod<1>(a, b); }
 // a b^+
-__device__ __forceinline__ M3 m3_muld(const M3 &a, const M3 &b) {
-  M3 r;
-#pragma unroll
-  for (int i = 0; i < 3; i++)
-#pragma unroll
-    for (int j = 0; j < 3; j++) {
-      double2 s = make_double2(0.0, 0.0);
-#pragma unroll
-      for (int k = 0; k < 3; k++) cfma(s, a.m[3 * i + k], cconj(b.m[3 * j + k]));
-      r.m[3 * i + j] = s;
-    }
-  return r;
-}
+__device__ __forceinline__ M3 m3_muld(const M3 &a, const M3 &b) { return m3_prod<2>(a, b); }
 __device__ __forceinline__ void m3_acc(M3 &a, const M3 &b) {
 #pragma unroll
   for (int k = 0; k < 9; k++) a.m[k] = cadd(a.m[k], b.m[k]);
@@ -87,6 +100,18 @@ __device__ __forceinline__ M3 link(const double2 *__restrict__ U, int mu, int eo
 #pragma unroll
   for (int k = 0; k < 9; k++) r.m[k] = b[(size_t)k * vh];
   return r;
+}
+// the same through a buffer resource over the (mu, eo) slab (mu, eo
+// wave-uniform): one 32-bit offset register per matrix instead of nine 64-bit
+// addresses; slab offsets stay below 2^32 bytes up to vh = 2^32 / 144
+__device__ __forceinline__ M3 link_b(const double2 *__restrict__ U, int mu, int eo, int i, int vh) {
+  const uint32_t plane = (uint32_t)vh * 16u;
+  const rsrc_t r = make_rsrc(U + (size_t)((mu * 2 + eo) * 9) * vh, plane * 9u);
+  const uint32_t vo = (uint32_t)i * 16u;
+  M3 m;
+#pragma unroll
+  for (int k = 0; k < 9; k++) m.m[k] = bld_f64x2(r, vo, plane * (uint32_t)k);
+  return m;
 }
 __device__ __forceinline__ void store_link(double2 *__restrict__ U, int mu, int eo, int i, int vh,
                                            const M3 &v) {
@@ -190,6 +215,36 @@ __device__ __forceinline__ void im_tr_j(double (&p)[8], const M3 &v, double s) {
   p[5] += s * (v.m[5].y + v.m[7].y);
   p[6] += s * (v.m[5].x - v.m[7].x);
   p[7] += s * (v.m[0].y + v.m[4].y - 2.0 * v.m[8].y) * is3;
+}
+
+// p_j += s * Re Tr(lambda_j a b) (MODE 0) or Re Tr(lambda_j a b^+) (MODE 2),
+// the elements of the product formed one at a time (the diagonal only in its
+// real part) instead of as a matrix: the clover force evaluates 24 such terms
+// per link and plane and is bound by registers and FP64 issue
+template <int MODE>
+__device__ __forceinline__ void re_tr_j_prod(double (&p)[8], const M3 &a, const M3 &b, double s) {
+  const double is3 = 0.57735026918962576;
+  {
+    const double d0 = m3_elem_re<MODE>(a, b, 0, 0), d1 = m3_elem_re<MODE>(a, b, 1, 1);
+    const double d2 = m3_elem_re<MODE>(a, b, 2, 2);
+    p[2] += s * (d0 - d1);
+    p[7] += s * (d0 + d1 - 2.0 * d2) * is3;
+  }
+  {
+    const double2 x01 = m3_elem<MODE>(a, b, 0, 1), x10 = m3_elem<MODE>(a, b, 1, 0);
+    p[0] += s * (x01.x + x10.x);
+    p[1] += s * (x10.y - x01.y);
+  }
+  {
+    const double2 x02 = m3_elem<MODE>(a, b, 0, 2), x20 = m3_elem<MODE>(a, b, 2, 0);
+    p[3] += s * (x02.x + x20.x);
+    p[4] += s * (x20.y - x02.y);
+  }
+  {
+    const double2 x12 = m3_elem<MODE>(a, b, 1, 2), x21 = m3_elem<MODE>(a, b, 2, 1);
+    p[5] += s * (x12.x + x21.x);
+    p[6] += s * (x21.y - x12.y);
+  }
 }
 
 // p_j += s * Re Tr(lambda_j v)  (su3.F90:216-233)
