@@ -82,6 +82,8 @@ def parse():
     ap.add_argument("--hmc-quench-beta", type=float, default=5.7)
     ap.add_argument("--hmc-therm-steps", type=int, default=20,
                     help="outer steps of each thermalisation trajectory")
+    ap.add_argument("--hmc-mre", type=int, default=4,
+                    help="chronological initial-guess vectors per pseudofermion (mre.F90)")
     ap.add_argument("--hmc-first", type=int, default=0,
                     help="index of the first measured trajectory (its draws and accept number): "
                          "a run split over several jobs continues where the last one stopped")
@@ -282,7 +284,7 @@ def hmc(args, torch, dev, stream):
     therm = bqcd_hip.hmc_params(beta=beta, kappa=kappa, csw=csw, rho=rho, tau=0.5 / args.hmc_therm_steps,
                                 ntau=args.hmc_therm_steps, m_scale=args.hmc_mscale, model="B",
                                 integrator="omelyan", rest=rest, maxiter=50000, precision="f16",
-                                mre_vectors=4, batch=True, verbose=True)
+                                mre_vectors=args.hmc_mre, batch=True, verbose=True)
     for k in range(args.hmc_therm):
         seed = args.seed + 7000 + 100 * k
         g.gaussian_d(p, seed + 1, 1.0)
@@ -304,7 +306,7 @@ def hmc(args, torch, dev, stream):
         prm = bqcd_hip.hmc_params(beta=beta, kappa=kappa, csw=csw, rho=rho, tau=1.0 / nsteps,
                                   ntau=nsteps, m_scale=args.hmc_mscale, model="B",
                                   integrator="omelyan", rest=rest, maxiter=50000, precision=prec,
-                                  mre_vectors=4, batch=batch, verbose=True)
+                                  mre_vectors=args.hmc_mre, batch=batch, verbose=True)
         trajs = []
         for k in range(args.hmc):
             seed = args.seed + 100 * (args.hmc_first + k)
@@ -333,7 +335,7 @@ def hmc(args, torch, dev, stream):
                        % "x".join(map(str, L)),
            "lattice": list(L), "beta": beta, "kappa": kappa, "csw": csw, "rho": rho,
            "integrator": "Omelyan (2MN), tau 1, %s outer steps, %d gauge steps each"
-                         % (args.hmc_steps, args.hmc_mscale),
+                         % (args.hmc_steps, args.hmc_mscale), "mre_vectors": args.hmc_mre,
            "trajectories": args.hmc, "first_trajectory": args.hmc_first, "thermalisation": "%s%d force-accepted trajectories of length 0.5 in %d steps"
                              % ("%d pure-gauge MD trajectories at beta %.3f, then " % (args.hmc_quench, args.hmc_quench_beta)
                                 if args.hmc_quench else "", args.hmc_therm, args.hmc_therm_steps), "runs": runs,

@@ -401,7 +401,9 @@ __global__ __launch_bounds__(128) void k_gauge_force(double *P, const double2 *_
     const int j4 = nb2(g, x[0], x[1], x[2], x[3], nu, -1, mu, 1);
     // U_nu(x+mu) U_mu(x+nu)^+ U_nu(x)^+  +  U_nu(x+mu-nu)^+ U_mu(x-nu)^+ U_nu(x-nu)
     m3_acc(st, m3_muld(m3_muld(link_b(U, nu, o, j1, vh), link_b(U, mu, o, j2, vh)), link_b(U, nu, e, i, vh)));
-    m3_acc(st, m3_mul(m3_dag(m3_mul(link_b(U, mu, o, j3, vh), link_b(U, nu, e, j4, vh))), link_b(U, nu, o, j3, vh)));
+    __builtin_amdgcn_sched_barrier(0);
+    m3_acc(st, m3_dmul(m3_mul(link_b(U, mu, o, j3, vh), link_b(U, nu, e, j4, vh)), link_b(U, nu, o, j3, vh)));
+    __builtin_amdgcn_sched_barrier(0);
   }
   double q[8] = {0, 0, 0, 0, 0, 0, 0, 0};
   im_tr_j(q, m3_mul(link_b(U, mu, e, i, vh), st), s);
