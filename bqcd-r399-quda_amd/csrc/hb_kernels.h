@@ -26,6 +26,10 @@ __device__ __forceinline__ rsrc_t make_rsrc(const void *p, uint32_t bytes) {
 // data read once per kernel, e.g. links, so that it does not evict the
 // neighbour spinors that other sites reuse)
 constexpr int kNT = 2;
+// load policy of the 16-bit links and clover words (kNT unless tuned otherwise)
+#ifndef HB_REDUCED_OPERATOR_POLICY
+#define HB_REDUCED_OPERATOR_POLICY kNT
+#endif
 template <int POL = 0>
 __device__ __forceinline__ double2 bld_f64x2(rsrc_t r, uint32_t vo, uint32_t so) {
   return __builtin_bit_cast(double2, __builtin_amdgcn_raw_buffer_load_b128(r, vo, so, POL));
@@ -189,6 +193,9 @@ __device__ __forceinline__ void load_spinor(typename S::V (&v)[12], const Spinor
   }
 }
 
+#ifndef HB_NT_STORE_REDUCED
+#define HB_NT_STORE_REDUCED 0
+#endif
 // FP64 results are written with non-temporal stores: an output field is read
 // again only by a later kernel, long after it would have left L2, so
 // streaming it past L2 keeps the L2 for the neighbour spinors the current
@@ -204,9 +211,17 @@ __device__ __forceinline__ void store_spinor(const SpinorPtr<S> &f, int i, int v
 #pragma unroll
     for (int k = 0; k < 12; k++) m = fmaxf(m, fmaxf(fabsf((float)v[k].x), fabsf((float)v[k].y)));
     float inv = m > 0.f ? 1.0f / m : 0.f;
+#if HB_NT_STORE_REDUCED
+    __builtin_nontemporal_store(m, &f.norm[i]);
+#pragma unroll
+    for (int k = 0; k < 12; k++)
+      __builtin_nontemporal_store(__builtin_bit_cast(uint32_t, S::qpack((float)v[k].x * inv, (float)v[k].y * inv)),
+                                  reinterpret_cast<uint32_t *>(f.p + (size_t)k * vh + i));
+#else
     f.norm[i] = m;
 #pragma unroll
     for (int k = 0; k < 12; k++) f.p[(size_t)k * vh + i] = S::qpack((float)v[k].x * inv, (float)v[k].y * inv);
+#endif
   } else if constexpr (sizeof(typename S::ST) == 16) {
 #pragma unroll
     for (int k = 0; k < 12; k++) S::st_nt(f.p, (size_t)k * vh + i, cmk<V>((A)v[k].x, (A)v[k].y));
@@ -252,7 +267,7 @@ __device__ __forceinline__ void load_link(typename S::V (&u)[9], const typename 
 // Other storages load the link as load_link and return 1.
 template <class G> constexpr bool link_raw() { return std::is_same<G, SF16>::value; }
 
-template <class G, int POL = kNT>
+template <class G, int POL = HB_REDUCED_OPERATOR_POLICY>
 __device__ __forceinline__ typename G::A load_link_f(typename G::V (&u)[9], const typename G::ST *U,
                                                      int mu, int eo, int i, int vh) {
   if constexpr (link_raw<G>()) {
@@ -489,7 +504,7 @@ __device__ __forceinline__ float2 load_clover_block_raw(float2 (&w)[18], const S
   rsrc_t r = make_rsrc(C + (size_t)(eo * 36 + J * 18) * vh, plane * 18u);
   const uint32_t vo = (uint32_t)i * SF16::kBytes;
 #pragma unroll
-  for (int k = 0; k < 18; k++) w[k] = SF16::bld_raw<kNT>(r, vo, plane * k);
+  for (int k = 0; k < 18; k++) w[k] = SF16::bld_raw<HB_REDUCED_OPERATOR_POLICY>(r, vo, plane * k);
   const float2 mc = reinterpret_cast<const float2 *>(Cn)[(size_t)(eo * 2 + J) * vh + i];
   return make_float2(mc.x * SF16::kInv, mc.y);
 }
