@@ -220,8 +220,9 @@ def batched(args, torch, dev, stream):
         return g.cg_multi_d("mtdagmt", xs, bs, kappas, csw=args.csw, rests=rests, maxiter=40000,
                             precision=prec, block=0)
 
-    block_solve()  # autotunes the RHS block size (cached per lattice and precision)
+    block_solve()  # autotunes the RHS block and workgroup sizes (cached per lattice and precision)
     t_blk, (its_blk, trs_blk, blk) = timed(block_solve)
+    tile = g.multi_tile()
     x_blk = [x.clone() for x in xs]
     t_seq_mixed, its_seq = timed(lambda: sequential(True))
     t_seq_f64, its_f64 = timed(lambda: sequential(False))
@@ -229,7 +230,7 @@ def batched(args, torch, dev, stream):
               for k in range(len(kappas)))
     res = {
         "lattice": list(L), "plaquette": round(plaq, 6), "nrhs": len(kappas), "kappas": kappas,
-        "csw": args.csw, "precision": prec, "block_size": blk,
+        "csw": args.csw, "precision": prec, "block_size": blk, "workgroup_size": tile,
         "block_solve_s": t_blk, "sequential_mixed_s": t_seq_mixed, "sequential_fp64_s": t_seq_f64,
         "speedup_vs_sequential_mixed": t_seq_mixed / t_blk,
         "speedup_vs_sequential_fp64": t_seq_f64 / t_blk,

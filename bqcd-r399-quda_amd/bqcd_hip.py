@@ -135,6 +135,8 @@ def lib():
                                     ctypes.POINTER(_vp), ctypes.POINTER(_vp), ctypes.POINTER(ctypes.c_float),
                                     ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, _ip, _dp]
         L.hb_multi_block_size.argtypes = [_vp]
+        L.hb_multi_tile.argtypes = [_vp]
+        L.hb_multi_set_tile.argtypes = [_vp, ctypes.c_int]
         L.hb_dsf.argtypes = [_vp, _dp, ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_double, _dp, _dp]
         L.hb_dsd.argtypes = [_vp, _dp, ctypes.c_int, ctypes.c_double, ctypes.c_double]
         L.hb_dsg.argtypes = [_vp, _dp, ctypes.c_int, ctypes.c_double, ctypes.c_double]
@@ -357,6 +359,14 @@ class Lattice:
         return xs, list(it), list(tr), lib().hb_multi_block_size(self._h)
 
     # ---- forces and link update (host arrays) ----------------------------------
+    def multi_tile(self):
+        """workgroup size of the block kernels of the last cg_multi solve"""
+        return lib().hb_multi_tile(self._h)
+
+    def set_multi_tile(self, tile):
+        """workgroup size (128 or 256) for cg_multi with an explicit block size"""
+        _check(lib().hb_multi_set_tile(self._h, tile))
+
     def gen(self, stride=None):
         return np.zeros((4, 2, stride or self.volh, 8))
 

@@ -152,6 +152,19 @@ int hb_multi_block_size(hb_context *ctx) {
   return c ? c->last_block : 0;
 }
 
+int hb_multi_tile(hb_context *ctx) {
+  Context *c = reinterpret_cast<Context *>(ctx);
+  return c ? c->last_tile : 0;
+}
+
+int hb_multi_set_tile(hb_context *ctx, int tile) {
+  Context *c = reinterpret_cast<Context *>(ctx);
+  HB_CHECK(c, HB_E_ARG, "hb_multi_set_tile: null context");
+  HB_CHECK(tile == 128 || tile == 256, HB_E_ARG, "hb_multi_set_tile: tile %d (use 128 or 256)", tile);
+  c->user_tile = tile;
+  return HB_OK;
+}
+
 int hb_last_timing(hb_context *ctx, double *solve_ms, int *n_matvec) {
   Context *c = reinterpret_cast<Context *>(ctx);
   HB_CHECK(c, HB_E_ARG, "hb_last_timing: null context");

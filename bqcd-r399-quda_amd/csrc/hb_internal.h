@@ -119,6 +119,9 @@ struct Context {
   double last_solve_ms = 0.0;
   int last_n_matvec = 0;
   int last_block = 0;                // RHS block size of the last multi-RHS solve
+  int multi_tile = 256;              // workgroup size of the block kernels
+  int last_tile = 0;                 // that of the last multi-RHS solve
+  int user_tile = 256;               // for solves with an explicit block size
   // chronological initial guesses (mre.F90) of the two pseudofermion solves
   // of the HMC: basis vectors live in scratch slots, newest first
   int mre_nvec = 0;

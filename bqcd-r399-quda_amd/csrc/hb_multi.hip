@@ -424,7 +424,7 @@ template <class G> size_t st_bytes() { return sizeof(typename G::ST); }
 template <class S, class G, int NR>
 int launch_multi_dslash(Context *c, const MultiArgs<S, G> &a, bool dag, int epi) {
   const int n = c->g.vh * NR;
-  dim3 grid(nblocks(n)), block(kBlock);
+  dim3 grid(nblocks(n, c->multi_tile)), block(c->multi_tile);
 #define HB_DM(D, E) hipLaunchKernelGGL((k_dslash_multi<S, G, NR, D, E>), grid, block, 0, c->stream, a)
   if (!dag) {
     if (epi == EPI_CLINV) HB_DM(false, EPI_CLINV);
@@ -485,9 +485,11 @@ int multi_apply(Context *c, const MultiFields<S, G> &f, int ngroups, const int *
   return HB_OK;
 }
 
-// per-block partial sums of the block kernels: NR values per block
+// per-block partial sums of the block kernels: NR values per block, sized for
+// the smallest workgroup the autotuner tries
+constexpr int kMinTile = 128;
 double *multi_partials(Context *c, int NR, int *rc) {
-  const size_t n = (size_t)NR * nblocks(c->g.vh * NR);
+  const size_t n = (size_t)NR * nblocks(c->g.vh * NR, kMinTile);
   return static_cast<double *>(scratch(c, SLOT_MULTI + 7, sizeof(double) * n, rc));
 }
 
@@ -602,10 +604,11 @@ int cg_multi_block(Context *c, const MultiPlan &plan, const MultiFields<S, G> &f
   // one batched iteration; captured once into a graph and replayed
   auto iteration = [&]() -> int {
     HB_TRY((multi_apply<S, G, NR>(c, f, plan.ngroups, grp, kap, rh, q, p, t, y, st, plan.clover)));
-    const int grid = nblocks(vh * NR);
-    k_multi_update<S, NR><<<grid, kBlock, 0, c->stream>>>(xlo, r, p, q, vh, st, part);
+    const int tile = c->multi_tile;
+    const int grid = nblocks(vh * NR, tile);
+    k_multi_update<S, NR><<<grid, tile, 0, c->stream>>>(xlo, r, p, q, vh, st, part);
     k_finalize<NR><<<1, kFinThreads, 0, c->stream>>>(part, grid, nullptr, FinMultiUpdate<NR>{st});
-    k_multi_pupdate<S, NR><<<grid, kBlock, 0, c->stream>>>(p, r, vh, st);
+    k_multi_pupdate<S, NR><<<grid, tile, 0, c->stream>>>(p, r, vh, st);
     HB_HIP(hipGetLastError());
     return HB_OK;
   };
@@ -668,14 +671,15 @@ int cg_multi_block(Context *c, const MultiPlan &plan, const MultiFields<S, G> &f
   return HB_OK;
 }
 
-// autotuner: seconds per RHS of one batched matvec for each block size
+// autotuner: seconds per RHS of one batched matvec for each block size and
+// workgroup size of the block kernels
 struct TuneKey {
   int L0, L1, L2, L3, prec, nrhs, ngroups;
   bool operator<(const TuneKey &o) const {
     return std::tie(L0, L1, L2, L3, prec, nrhs, ngroups) < std::tie(o.L0, o.L1, o.L2, o.L3, o.prec, o.nrhs, o.ngroups);
   }
 };
-static std::map<TuneKey, int> g_tuned;
+static std::map<TuneKey, std::pair<int, int>> g_tuned;  // (block, tile)
 
 template <class S, class G, int NR>
 int time_block(Context *c, const MultiFields<S, G> &f, const MultiPlan &plan, double *ms_per_rhs) {
@@ -721,40 +725,51 @@ int cg_multi_prec(Context *c, MultiPlan &plan, int op, int nrhs, const double *k
   MultiFields<S, G> f;
   HB_TRY((multi_fields<S, G>(c, plan, f)));
   static const int cand[5] = {1, 2, 4, 8, 12};
+  static const int tiles[2] = {kMinTile, 256};
+  int tile = c->user_tile;
   if (block == 0) {
     TuneKey key{c->g.L[0], c->g.L[1], c->g.L[2], c->g.L[3], prec, nrhs, plan.ngroups};
     auto it = g_tuned.find(key);
     if (it != g_tuned.end()) {
-      block = it->second;
+      block = it->second.first;
+      tile = it->second.second;
     } else {
       double best = 1e30;
       for (int nr : cand) {
         if (nr > nrhs && nr != 1) break;
-        double ms = 0.0;
-        int r;
-        switch (nr) {
-          case 1: r = time_block<S, G, 1>(c, f, plan, &ms); break;
-          case 2: r = time_block<S, G, 2>(c, f, plan, &ms); break;
-          case 4: r = time_block<S, G, 4>(c, f, plan, &ms); break;
-          case 8: r = time_block<S, G, 8>(c, f, plan, &ms); break;
-          default: r = time_block<S, G, 12>(c, f, plan, &ms); break;
-        }
-        HB_TRY(r);
-        // the partial last block costs as much as a full one
-        const int nblk = (nrhs + nr - 1) / nr;
-        const double total = ms * nr * nblk;
-        if (total < best) {
-          best = total;
-          block = nr;
+        for (int tl : tiles) {
+          c->multi_tile = tl;
+          double ms = 0.0;
+          int r;
+          switch (nr) {
+            case 1: r = time_block<S, G, 1>(c, f, plan, &ms); break;
+            case 2: r = time_block<S, G, 2>(c, f, plan, &ms); break;
+            case 4: r = time_block<S, G, 4>(c, f, plan, &ms); break;
+            case 8: r = time_block<S, G, 8>(c, f, plan, &ms); break;
+            default: r = time_block<S, G, 12>(c, f, plan, &ms); break;
+          }
+          c->multi_tile = 256;
+          HB_TRY(r);
+          // the partial last block costs as much as a full one
+          const int nblk = (nrhs + nr - 1) / nr;
+          const double total = ms * nr * nblk;
+          if (total < best) {
+            best = total;
+            block = nr;
+            tile = tl;
+          }
         }
       }
-      g_tuned[key] = block;
+      g_tuned[key] = {block, tile};
     }
   }
   HB_CHECK(block == 1 || block == 2 || block == 4 || block == 8 || block == 12, HB_E_ARG,
            "cg_multi: block size %d (use 0, 1, 2, 4, 8 or 12)", block);
   c->last_block = block;
-  for (int k0 = 0; k0 < nrhs; k0 += block) {
+  c->last_tile = tile;
+  c->multi_tile = tile;
+  int result = HB_OK;
+  for (int k0 = 0; k0 < nrhs && result == HB_OK; k0 += block) {
     std::vector<int> idx;
     for (int k = k0; k < std::min(nrhs, k0 + block); k++) idx.push_back(k);
     int r;
@@ -765,9 +780,10 @@ int cg_multi_prec(Context *c, MultiPlan &plan, int op, int nrhs, const double *k
       case 8: r = cg_multi_block<S, G, 8>(c, plan, f, op, idx, kappa, rho, x, b, rest, maxiter, delta, max_inner, iters, true_rtr, nullptr); break;
       default: r = cg_multi_block<S, G, 12>(c, plan, f, op, idx, kappa, rho, x, b, rest, maxiter, delta, max_inner, iters, true_rtr, nullptr); break;
     }
-    HB_TRY(r);
+    result = r;
   }
-  return HB_OK;
+  c->multi_tile = 256;
+  return result;
 }
 
 int cg_multi(Context *c, int op, int nrhs, const double *kappa, const double *rho, double csw,

@@ -178,6 +178,15 @@ module bqcd_hip
        import :: c_int, c_ptr
        type(c_ptr), value :: ctx
      end function
+     integer(c_int) function hb_multi_tile(ctx) bind(C, name="hb_multi_tile")
+       import :: c_int, c_ptr
+       type(c_ptr), value :: ctx
+     end function
+     integer(c_int) function hb_multi_set_tile(ctx, tile) bind(C, name="hb_multi_set_tile")
+       import :: c_int, c_ptr
+       type(c_ptr), value :: ctx
+       integer(c_int), value :: tile
+     end function
 
      integer(c_int) function hb_dsf(ctx, p, site_stride, step, kappa, csw_kappa, a, b) &
           bind(C, name="hb_dsf")

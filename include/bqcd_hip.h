@@ -129,12 +129,19 @@ int hb_cg_mixed(hb_context *ctx, int op, double kappa, double rho, double *x, co
  * iterations in precision (HB_PREC_F32 or HB_PREC_F16), FP64 reliable updates
  * per system; each stops on its FP64 true residual <= rest[k].  block = RHS
  * block size 1, 2, 4, 8 or 12, or 0 to let the autotuner time the candidates
- * and choose.  iters[k] and true_rtr[k] are outputs. */
+ * (block size and workgroup size of the block kernels) and choose; an explicit
+ * block size runs with 256-thread workgroups.  iters[k] and true_rtr[k] are
+ * outputs. */
 int hb_cg_multi(hb_context *ctx, int op, int nrhs, const double *kappa, const double *rho,
                 double csw, double *const *x, const double *const *b, const float *rest,
                 int maxiter, int log, int precision, int block, int *iters, double *true_rtr);
 /* block size chosen by the last hb_cg_multi call */
 int hb_multi_block_size(hb_context *ctx);
+/* workgroup size (threads) of the block kernels of the last hb_cg_multi call */
+int hb_multi_tile(hb_context *ctx);
+/* workgroup size for hb_cg_multi calls with an explicit block size: 128 or 256
+ * (the default); the autotuner (block = 0) chooses its own */
+int hb_multi_set_tile(hb_context *ctx, int tile);
 
 /* ---- forces (host, BQCD layout; p accumulates) -------------------------- */
 /* dsf (dsf.F90:28-110): hopping + clover part of dS_f/dU for spinors a, b

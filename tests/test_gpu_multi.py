@@ -62,6 +62,30 @@ def test_multi_autotune_wdw(setup, prec):
     check(lat, refs, xs, its, trs, phis, kap, rests, 5)
 
 
+@pytest.mark.parametrize("block", [4, 12])
+def test_multi_tile_128(setup, block):
+    """the 128-thread workgroups the autotuner may choose give the same solutions"""
+    g, lat, refs, phis, kappas = setup
+    kap = [kappas[j % 3] for j in range(12)]
+    rests = [1e-20 * float(np.sum(p * p)) for p in phis]
+    g.set_multi_tile(128)
+    try:
+        xs, its, trs, used = g.cg_multi("mtdagmt", phis, kap, csw=CSW, rests=rests, block=block,
+                                        precision="f16")
+        assert g.multi_tile() == 128
+    finally:
+        g.set_multi_tile(256)
+    assert used == block
+    check(lat, refs, xs, its, trs, phis, kap, rests, 2)
+
+
+def test_multi_tile_rejects_other_sizes(setup):
+    import bqcd_hip
+    g = setup[0]
+    with pytest.raises(bqcd_hip.HipError, match="tile"):
+        g.set_multi_tile(64)
+
+
 def test_multi_with_loaded_clover_single_kappa(setup):
     g, lat, refs, phis, kappas = setup
     u = g.save_gauge()
