@@ -58,6 +58,9 @@ def parse():
     ap.add_argument("--seed", type=int, default=20261202)
     ap.add_argument("--sweeps", type=int, default=5)
     ap.add_argument("--alpha", type=float, default=0.36)
+    ap.add_argument("--only-mixed", type=int, default=0, metavar="MAXITER",
+                    help="run only one mixed-precision solve (--solver) stopped after MAXITER inner "
+                         "iterations (for hardware-counter runs), then exit")
     ap.add_argument("--only-dslash", type=int, default=0,
                     help="run only N FP64 hopping-matrix launches and a calibration copy of the "
                          "same byte count (for hardware-counter runs), then exit")
@@ -456,6 +459,14 @@ def main():
                             delta=args.delta, max_inner=args.max_inner)
 
     solve = solve_f64 if args.solver == "f64" else solve_mixed
+    if args.only_mixed:
+        x.zero_()
+        it, nrel, tr = g.cg_mixed_d("mtdagmt", x, b, kappa, rest=rest, maxiter=args.only_mixed, log=2,
+                                    precision="f16" if args.solver != "mixed32" else "f32",
+                                    delta=args.delta, max_inner=args.max_inner)
+        torch.cuda.synchronize()
+        log("bounded mixed solve: %d iterations, %d reliable updates" % (it, nrel))
+        return 0
 
     # FP64 reference solve: time to solution of the plain FP64 GPU path, and
     # the solution the mixed solver is compared with
