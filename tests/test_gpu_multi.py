@@ -95,3 +95,32 @@ def test_multi_with_loaded_clover_single_kappa(setup):
     rests = [1e-20 * float(np.sum(p * p)) for p in phis[:3]]
     xs, its, trs, used = g.cg_multi("mtdagmt", phis[:3], kap, csw=-1.0, rests=rests, block=2)
     check(lat, refs, xs, its, trs, phis, kap, rests, 2)
+
+
+@pytest.mark.parametrize("L", [(6, 4, 6, 6), (10, 6, 6, 12)])
+@pytest.mark.parametrize("tile", [128, 256])
+def test_multi_ragged_lattice_partial_blocks(L, tile):
+    """site counts that are not multiples of the workgroup (partial last blocks in
+    every block kernel), three kappa groups in one block of 4 and a block of 12"""
+    import bqcd_hip
+    lat = ol.Lattice(L)
+    u = lat.gauge_exp_random(77, 0.3)
+    kappas = [0.125, 0.12, 0.11]
+    refs = {}
+    for k in kappas:
+        a, ai, b = lat.clover_init(u, CSW * k)
+        refs[k] = lat.op(lat.flip_bc(u), a, ai, k, CSW * k, 0.0)
+    phis = [lat.spinor_gauss(300 + j, 0) for j in range(12)]
+    kap = [kappas[j % 3] for j in range(12)]
+    rests = [1e-20 * float(np.sum(p * p)) for p in phis]
+    g = bqcd_hip.Lattice(L)
+    try:
+        g.load_gauge(u)
+        g.set_multi_tile(tile)
+        for block in (4, 12):
+            xs, its, trs, used = g.cg_multi("mtdagmt", phis, kap, csw=CSW, rests=rests, block=block,
+                                            precision="f16")
+            assert used == block and g.multi_tile() == tile
+            check(lat, refs, xs, its, trs, phis, kap, rests, 2)
+    finally:
+        g.close()
