@@ -7,7 +7,9 @@ here by the oracle's ranf, which reproduces ranf_test.reference exactly):
    MRE with 7 vectors, solver bound 1e-5); its golden value is the one the
    reference itself printed (work/out.quick-test.reference:44);
  - traj_b: model B on a rough start, tight solver, no MRE;
- - traj_a: model A with MRE 3.
+ - traj_a: model A with MRE 3;
+ - traj_ragged: model B with MRE 4 on a 6x4x2x8 lattice (every extent
+   different, partial workgroups).
 Bars: exp(-Delta H) to 1e-9 relative, final plaquette action and links to
 1e-10, identical CG call and iteration counts.
 
@@ -49,7 +51,7 @@ def params(hip, f, **kw):
     return hip.hmc_params(**d)
 
 
-@pytest.mark.parametrize("case", ["traj_quick", "traj_b", "traj_a"])
+@pytest.mark.parametrize("case", ["traj_quick", "traj_b", "traj_a", "traj_ragged"])
 def test_trajectory_vs_reference(hip, golden, case):
     f = golden(case)
     L = tuple(int(x) for x in f["L"])
