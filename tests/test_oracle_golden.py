@@ -14,7 +14,7 @@ import pytest
 import oracle_lib as ol
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-TOL = 1e-13  # relative to the field's max magnitude, per north-star parity bar
+TOL = 1e-13  # relative to the field's max magnitude: the parity bar of the FP64 operator
 
 
 def rel(a, b):
