@@ -373,8 +373,8 @@ def scan(g, args, torch, dev):
 def main():
     args = parse()
     if args.gpus != 1:
-        print("bench.py measures one GPU: this build runs on a single MI355X "
-              "(multi-GPU sharding is a design, not part of this build); use --gpus 1",
+        print("bench.py measures one GPU: the library runs on a single MI355X "
+              "(multi-GPU sharding is designed but not implemented); use --gpus 1",
               file=sys.stderr)
         return 2
     import torch
