@@ -98,3 +98,23 @@ def test_corrupt_and_mismatched_files_are_rejected(ref):
         open(slice3, "wb").write(bytes(b))
         with pytest.raises(bqcd_hip.HipError, match="checksum"):
             bqcd_hip.conf_read(info, L)
+
+
+def test_missing_truncated_and_unwritable_files_are_errors(ref):
+    import bqcd_hip
+
+    L, u, files = ref
+    slices = sorted(n for n in files if n.endswith(".u"))
+    with tempfile.TemporaryDirectory() as d:
+        info = unpack(files, d)
+        os.remove(os.path.join(d, slices[2]))
+        with pytest.raises(bqcd_hip.HipError, match="cannot open"):
+            bqcd_hip.conf_read(info, L)
+        unpack(files, d)
+        with open(os.path.join(d, slices[4]), "wb") as fh:
+            fh.write(files[slices[4]][:100])
+        with pytest.raises(bqcd_hip.HipError, match="wrong size"):
+            bqcd_hip.conf_read(info, L)
+        with pytest.raises(bqcd_hip.HipError, match="cannot write"):
+            bqcd_hip.conf_write(L, u, os.path.join(d, "no", "such", "dir"), run=1, traj=0,
+                                beta=5.29, kappa=0.13, csw=1.9)
