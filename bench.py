@@ -30,6 +30,10 @@ REPO = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(REPO, "bqcd-r399-quda_amd"))
 
 DSLASH_BYTES_PER_SITE = 8 * 144 + 192 + 192
+# 16-bit hopping kernel with the A^-1 epilogue (first half of the mixed solver's
+# operator): 8 two-row links of 24 B, the input and output spinors (48 B + 4 B
+# norm each), two clover blocks of 18 words (72 B) plus their (m, c) float2
+DSLASH16_CLINV_BYTES_PER_SITE = 8 * 24 + 2 * (48 + 4) + 2 * (72 + 8)
 DSLASH_FLOPS_PER_SITE = 1320
 HBM_PEAK_GBS = 8000.0
 
@@ -443,6 +447,10 @@ def main():
             dst.copy_(src)
         torch.cuda.synchronize()
         log("calibration copy: %d bytes read + %d bytes written per launch" % (ds_bytes // 2, ds_bytes // 2))
+        # the 16-bit hopping kernel of the mixed solver (A^-1 D), timed the same way
+        ms16 = g.time_dslash_lp(b, "f16", reps=args.dslash_reps)
+        log("16-bit dslash (A^-1 D): %.4f ms/launch, %.1f GB/s algorithmic"
+            % (ms16, DSLASH16_CLINV_BYTES_PER_SITE * vh / (ms16 * 1e-3) / 1e9))
         return 0
     del out, inp
 
@@ -509,6 +517,22 @@ def main():
         "|x-x64|/|x64| = %s" % (args.solver, ms, iters[0], nrels[0], true_rel, sol_err))
     if true_rel > 1e-20 * 1.01:
         log("WARNING: true residual above 1e-20 |b|^2")
+
+    # the dominant kernel of the mixed solve: the 16-bit hopping kernel, timed in
+    # isolation with HIP events on the context stream
+    roof16 = None
+    if args.solver == "mixed16":
+        ms16 = g.time_dslash_lp(b, "f16", reps=args.dslash_reps)
+        b16 = DSLASH16_CLINV_BYTES_PER_SITE * vh
+        gbs16 = b16 / (ms16 * 1e-3) / 1e9
+        log("16-bit dslash (A^-1 D): %.4f ms/launch, %.1f GB/s algorithmic" % (ms16, gbs16))
+        roof16 = {"bound": "hbm", "achieved": gbs16, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                  "frac": gbs16 / HBM_PEAK_GBS, "traffic": None,
+                  "kernel": "k_dslash<SF16, SF16, false, EPI_CLINV> (16-bit hopping matrix with "
+                            "the A^-1 epilogue; 4 such launches per inner iteration)",
+                  "bytes_per_launch": b16, "ms_per_launch": ms16,
+                  "note": "bound by the vector memory path, not HBM "
+                          "(profiles/dslash_counters_32x32x32x64.md)"}
 
     # precision comparison (FP32 inner, then 16-bit storage) at the reliable-update
     # settings the precision configuration prescribes: update every 100 inner
@@ -592,6 +616,7 @@ def main():
             "kernel": "k_dslash<SF64, false, EPI_PLAIN> (FP64 hopping matrix, one parity)",
             "bytes_per_launch": ds_bytes,
         },
+        "roofline_mixed16": roof16,
         "cpu_baseline": cpu,
         "precision_comparison": sweep,
         "batched": batch,

@@ -136,6 +136,7 @@ def lib():
                                     ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, _ip, _dp]
         L.hb_multi_block_size.argtypes = [_vp]
         L.hb_multi_tile.argtypes = [_vp]
+        L.hb_time_dslash_lp.argtypes = [_vp, ctypes.c_int, ctypes.c_int, _vp, _dp]
         L.hb_multi_set_tile.argtypes = [_vp, ctypes.c_int]
         L.hb_dsf.argtypes = [_vp, _dp, ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_double, _dp, _dp]
         L.hb_dsd.argtypes = [_vp, _dp, ctypes.c_int, ctypes.c_double, ctypes.c_double]
@@ -359,6 +360,13 @@ class Lattice:
         return xs, list(it), list(tr), lib().hb_multi_block_size(self._h)
 
     # ---- forces and link update (host arrays) ----------------------------------
+    def time_dslash_lp(self, inp, precision="f16", reps=20):
+        """average ms of the reduced-precision hopping kernel (A^-1 D, odd output) of
+        the mixed solver on the even device spinor inp (12, volh, 2)"""
+        ms = ctypes.c_double(0.0)
+        _check(lib().hb_time_dslash_lp(self._h, PREC[precision], reps, _tp(inp), ctypes.byref(ms)))
+        return ms.value
+
     def multi_tile(self):
         """workgroup size of the block kernels of the last cg_multi solve"""
         return lib().hb_multi_tile(self._h)

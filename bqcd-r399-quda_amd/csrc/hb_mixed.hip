@@ -442,11 +442,62 @@ int cg_mixed(Context *c, int op, double kappa, double rho, double2 *x, const dou
   return HB_E_ARG;
 }
 
+// average time of the first half of the reduced-precision operator, the
+// hopping kernel with the A_oo^-1 epilogue (out = A^-1 D in, odd output), on the
+// context stream
+template <class S>
+int time_dslash_lp(Context *c, int reps, const double2 *in64, double *ms_out) {
+  int rc = 0;
+  SpinorPtr<S> in = SpinorAlloc<S>::get(c, SLOT_LP0, SLOT_LPN, &rc); HB_TRY(rc);
+  SpinorPtr<S> out = SpinorAlloc<S>::get(c, SLOT_LP3, SLOT_LPN3, &rc); HB_TRY(rc);
+  k_cvt_spinor<S><<<nblocks(c->g.vh), kBlock, 0, c->stream>>>(in, in64, c->g.vh);
+  HB_HIP(hipGetLastError());
+  const bool clov = use_clover(c);
+  DslashArgs<S, S> a{};
+  a.g = c->g;
+  a.out_parity = 1;
+  a.out = out;
+  a.in = in;
+  a.U = LP<S>::gauge(c);
+  a.C = clov ? LP<S>::clov(c, HB_CLOVER_AINV) : nullptr;
+  a.Cn = clov ? LP<S>::norm(c, HB_CLOVER_AINV) : nullptr;
+  for (int k = 0; k < 2; k++) HB_TRY(launch_dslash(a, false, EPI_CLINV, c->stream));
+  hipEvent_t e0, e1;
+  HB_HIP(hipEventCreate(&e0));
+  HB_HIP(hipEventCreate(&e1));
+  HB_HIP(hipEventRecord(e0, c->stream));
+  int r = HB_OK;
+  for (int k = 0; k < reps && r == HB_OK; k++) r = launch_dslash(a, false, EPI_CLINV, c->stream);
+  HB_HIP(hipEventRecord(e1, c->stream));
+  HB_HIP(hipEventSynchronize(e1));
+  float ms = 0.f;
+  HB_HIP(hipEventElapsedTime(&ms, e0, e1));
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  HB_TRY(r);
+  *ms_out = ms / reps;
+  return HB_OK;
+}
+
 }  // namespace hb
 
 using namespace hb;
 
 extern "C" {
+
+int hb_time_dslash_lp(hb_context *ctx, int precision, int reps, const double *in_d, double *ms) {
+  Context *c = reinterpret_cast<Context *>(ctx);
+  HB_TRY(check_ready(c));
+  HB_CHECK(in_d && ms, HB_E_ARG, "hb_time_dslash_lp: null pointer");
+  HB_CHECK(reps >= 1, HB_E_ARG, "hb_time_dslash_lp: reps must be >= 1");
+  HB_CHECK(precision == HB_PREC_F32 || precision == HB_PREC_F16, HB_E_ARG,
+           "hb_time_dslash_lp: precision %d (use 32 or 16)", precision);
+  HB_HIP(hipSetDevice(c->device));
+  HB_TRY(ensure_lowprec(c, precision));
+  const double2 *in = reinterpret_cast<const double2 *>(in_d);
+  return precision == HB_PREC_F16 ? time_dslash_lp<SF16>(c, reps, in, ms)
+                                  : time_dslash_lp<SF32>(c, reps, in, ms);
+}
 
 int hb_cg_mixed_d(hb_context *ctx, int op, double kappa, double rho, double *x_d, const double *b_d,
                   float rest, int maxiter, int log, int precision, double delta, int max_inner,

@@ -182,6 +182,12 @@ module bqcd_hip
        import :: c_int, c_ptr
        type(c_ptr), value :: ctx
      end function
+     integer(c_int) function hb_time_dslash_lp(ctx, precision, reps, in_d, ms) bind(C, name="hb_time_dslash_lp")
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: ctx, in_d
+       integer(c_int), value :: precision, reps
+       real(c_double) :: ms
+     end function
      integer(c_int) function hb_multi_set_tile(ctx, tile) bind(C, name="hb_multi_set_tile")
        import :: c_int, c_ptr
        type(c_ptr), value :: ctx

@@ -194,6 +194,12 @@ int hb_gauge_smear(hb_context *ctx, int sweeps, double alpha);
 /* timing of the last solve: kernel time (ms) of the hopping-matrix launches
  * and their count, measured with HIP events on the context stream */
 int hb_last_timing(hb_context *ctx, double *solve_ms, int *n_matvec);
+/* average time (ms, HIP events on the context stream) of reps launches of the
+ * reduced-precision hopping kernel of the mixed solver's operator (out = A^-1 D
+ * in on the odd sites), precision HB_PREC_F16 or HB_PREC_F32, on the even
+ * device spinor in_d (converted first); a benchmarking aid with no counterpart
+ * in the reference */
+int hb_time_dslash_lp(hb_context *ctx, int precision, int reps, const double *in_d, double *ms);
 
 /* ---- HMC trajectory: replaces hmc() (hmc.F90:28-180) with the integrator
  *      of hmc_leap_frog.F90:28-79 / hmc_integrator.F90:28-103 ------------- */

@@ -53,3 +53,17 @@ def test_mixed_from_guess_and_maxiter(setup):
     import bqcd_hip
     with pytest.raises(bqcd_hip.NoConvergence):
         g.cg_mixed("mtdagmt", phi, kappa, rest=1e-40, maxiter=20, precision="f16")
+
+
+def test_time_dslash_lp(setup):
+    """the benchmarking entry point times the reduced-precision hopping kernel"""
+    import bqcd_hip
+    import torch
+    g = setup[0] if isinstance(setup, tuple) else setup
+    dev = torch.device("cuda:0")
+    b = torch.randn((12, g.volh, 2), dtype=torch.float64, device=dev)
+    for prec in ("f16", "f32"):
+        ms = g.time_dslash_lp(b, prec, reps=3)
+        assert 0.0 < ms < 1000.0
+    with pytest.raises(bqcd_hip.HipError, match="precision"):
+        g.time_dslash_lp(b, "f64", reps=1)
