@@ -191,8 +191,8 @@ int hb_hmc_u_d(hb_context *ctx, const double *p_d, double step, double csw_kappa
 int hb_gauge_random(hb_context *ctx, unsigned long long seed, double scale);
 int hb_gauge_smear(hb_context *ctx, int sweeps, double alpha);
 
-/* timing of the last solve: kernel time (ms) of the hopping-matrix launches
- * and their count, measured with HIP events on the context stream */
+/* timing of the last solve: its time (ms) between HIP events recorded on the
+ * context stream at its start and end, and its number of operator applications */
 int hb_last_timing(hb_context *ctx, double *solve_ms, int *n_matvec);
 /* average time (ms, HIP events on the context stream) of reps launches of the
  * reduced-precision hopping kernel of the mixed solver's operator (out = A^-1 D
