@@ -13,6 +13,7 @@
 #include <algorithm>
 #include <cmath>
 #include <map>
+#include <mutex>
 #include <vector>
 
 #include "hb_hop.h"
@@ -680,6 +681,7 @@ struct TuneKey {
   }
 };
 static std::map<TuneKey, std::pair<int, int>> g_tuned;  // (block, tile)
+static std::mutex g_tuned_mutex;                        // contexts may be used from several threads
 
 template <class S, class G, int NR>
 int time_block(Context *c, const MultiFields<S, G> &f, const MultiPlan &plan, double *ms_per_rhs) {
@@ -729,11 +731,17 @@ int cg_multi_prec(Context *c, MultiPlan &plan, int op, int nrhs, const double *k
   int tile = c->user_tile;
   if (block == 0) {
     TuneKey key{c->g.L[0], c->g.L[1], c->g.L[2], c->g.L[3], prec, nrhs, plan.ngroups};
-    auto it = g_tuned.find(key);
-    if (it != g_tuned.end()) {
-      block = it->second.first;
-      tile = it->second.second;
-    } else {
+    bool cached = false;
+    {
+      std::lock_guard<std::mutex> lock(g_tuned_mutex);
+      auto it = g_tuned.find(key);
+      if (it != g_tuned.end()) {
+        block = it->second.first;
+        tile = it->second.second;
+        cached = true;
+      }
+    }
+    if (!cached) {
       double best = 1e30;
       for (int nr : cand) {
         if (nr > nrhs && nr != 1) break;
@@ -760,6 +768,7 @@ int cg_multi_prec(Context *c, MultiPlan &plan, int op, int nrhs, const double *k
           }
         }
       }
+      std::lock_guard<std::mutex> lock(g_tuned_mutex);
       g_tuned[key] = {block, tile};
     }
   }

@@ -5,7 +5,9 @@
  * fermion / clover-determinant / gauge forces.
  *
  * Plain C types only.  Every entry point returns 0 on success or a negative
- * HB_E_* code; hb_last_error() gives a message.  A Fortran iso_c_binding
+ * HB_E_* code; hb_last_error() gives the calling thread's last message.  A
+ * context is used by one host thread at a time (its calls are not re-entrant);
+ * distinct contexts may be used from different threads.  A Fortran iso_c_binding
  * module (bqcd-r399-quda_amd/fortran/bqcd_hip.F90) declares every entry
  * point; bqcd-r399-quda_amd/fortran/hip_solver.F90 maps them onto BQCD's
  * cg() / quda_solver(), dsf, dsd, dsg and hmc_u interfaces.
