@@ -67,3 +67,33 @@ def test_time_dslash_lp(setup):
         assert 0.0 < ms < 1000.0
     with pytest.raises(bqcd_hip.HipError, match="precision"):
         g.time_dslash_lp(b, "f64", reps=1)
+
+
+@pytest.mark.parametrize("prec", ["f32", "f16"])
+def test_mixed_and_multi_without_clover(prec):
+    """Wilson fermions (csw = 0: no clover term) through the reduced-precision
+    single- and multi-RHS solvers, against the oracle's FP64 solve"""
+    import bqcd_hip
+    L = (6, 4, 4, 8)
+    lat = ol.Lattice(L)
+    u = lat.gauge_exp_random(91, 0.3)
+    kappa = 0.12
+    op = lat.op(lat.flip_bc(u), lat.clover(), lat.clover(), kappa, 0.0, 0.0)
+    phis = [lat.spinor_gauss(400 + j, 0) for j in range(3)]
+    rests = [1e-20 * float(np.sum(p * p)) for p in phis]
+    g = bqcd_hip.Lattice(L)
+    try:
+        g.load_gauge(u)
+        x, it, nrel, tr = g.cg_mixed("mtdagmt", phis[0], kappa, rest=rests[0], maxiter=5000,
+                                     precision=prec)
+        xr, _, _ = lat.cg(op, 2, phis[0], rests[0] * 1e-4, 5000)
+        assert tr <= rests[0]
+        assert np.linalg.norm(x - xr) / np.linalg.norm(xr) < 1e-7
+        xs, its, trs, used = g.cg_multi("mtdagmt", phis, [kappa] * 3, csw=0.0, rests=rests, block=4,
+                                        precision=prec)
+        for k in range(3):
+            xr, _, _ = lat.cg(op, 2, phis[k], rests[k] * 1e-4, 5000)
+            assert trs[k] <= rests[k]
+            assert np.linalg.norm(xs[k] - xr) / np.linalg.norm(xr) < 1e-7
+    finally:
+        g.close()
