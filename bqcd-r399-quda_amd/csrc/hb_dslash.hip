@@ -35,7 +35,17 @@ __global__ __launch_bounds__(kBlock) void k_dslash(DslashArgs<S, G> a) {
     int xc[4];
     site_coords(g, e, i, xc[0], xc[1], xc[2], xc[3]);
     const SpinorView<S> in(a.in, vh);
-    hop_all<S, G, DAG>(acc, a.U, in, g, e, i, xc);
+    if constexpr (HB_WAVE_SHARE && std::is_same<S, SF16>::value) {
+      uint32_t w[13];
+      const uint32_t vo = (uint32_t)i * 4u;
+#pragma unroll
+      for (int k = 0; k < 12; k++) w[k] = __builtin_amdgcn_raw_buffer_load_b32(in.r, vo, in.plane * k, 0);
+      w[12] = __builtin_amdgcn_raw_buffer_load_b32(in.rn, vo, 0, 0);
+      const WaveView16 wv{in, w, i - (int)(threadIdx.x & 63), g.L[0] / 2 <= 32};
+      hop_all<S, G, DAG, 1, WaveView16>(acc, a.U, wv, g, e, i, xc);
+    } else {
+      hop_all<S, G, DAG>(acc, a.U, in, g, e, i, xc);
+    }
   }
 
   if constexpr (EPI == EPI_PLAIN) {

@@ -10,9 +10,9 @@ namespace hb {
 // U_mu(x - mu)^+ backward, with the boundary sign when the hop wraps.
 // For a block of NR right-hand sides stored RHS-fastest, the spinor of site j
 // for RHS r sits at index j*NR + r of the view (NR = 1: plain layout).
-template <class S, class G, int MU, bool FWD, int SG, int NR = 1>
+template <class S, class G, int MU, bool FWD, int SG, int NR = 1, class View = SpinorView<S>>
 __device__ __forceinline__ void hop(typename S::V (&acc)[12], const typename G::ST *U,
-                                    const SpinorView<S> &in, const Geom &g, int e, int i,
+                                    const View &in, const Geom &g, int e, int i,
                                     const int (&xc)[4], int r = 0) {
   using V = typename S::V;
   using A = typename S::A;
@@ -42,7 +42,7 @@ __device__ __forceinline__ void hop(typename S::V (&acc)[12], const typename G::
       }
     } else {
       V p[12];
-      f = sgn * in.load_raw(p, j * NR + r);
+      f = sgn * in.template load_raw_dir<MU>(p, j * NR + r);
       project<MU, SG>(h, p);
     }
     V u[9];
@@ -99,6 +99,11 @@ template <class V> __device__ __forceinline__ void pin(V (&acc)[12]) {
 // issue two (the +mu and -mu hops), which overlaps one hop's arithmetic with
 // the other's loads (2% on the 16-bit solver; four or eight hops gain no more
 // and eight costs occupancy).
+// the 16-bit hopping kernels take the x and y neighbours from other lanes of
+// the wave where they are there (WaveView16)
+#ifndef HB_WAVE_SHARE
+#define HB_WAVE_SHARE 1
+#endif
 #ifndef HB_HOPS_PER_FENCE_REDUCED
 #define HB_HOPS_PER_FENCE_REDUCED 2
 #endif
@@ -109,15 +114,15 @@ template <class S> constexpr int hops_per_fence() {
 // all 8 hops of the hopping matrix into acc, t first (the reference's
 // direction order does not matter for the sum); the forward projector sign is
 // (1 - gamma) for D and (1 + gamma) for D^+
-template <class S, class G, bool DAG, int NR = 1>
+template <class S, class G, bool DAG, int NR = 1, class View = SpinorView<S>>
 __device__ __forceinline__ void hop_all(typename S::V (&acc)[12], const typename G::ST *U,
-                                        const SpinorView<S> &in, const Geom &g, int e, int i,
+                                        const View &in, const Geom &g, int e, int i,
                                         const int (&xc)[4], int r = 0) {
   constexpr int F = hops_per_fence<S>();
   constexpr int SF = DAG ? 1 : -1;
   constexpr int SB = -SF;
 #define HB_HOP(MU, FWD, SG, N)                                    \
-  hop<S, G, MU, FWD, SG, NR>(acc, U, in, g, e, i, xc, r);         \
+  hop<S, G, MU, FWD, SG, NR, View>(acc, U, in, g, e, i, xc, r);   \
   if constexpr ((N) % F == 0) HB_SCHED_FENCE();
   HB_HOP(3, true, SF, 1) HB_HOP(3, false, SB, 2)
   HB_HOP(2, true, SF, 3) HB_HOP(2, false, SB, 4)

@@ -174,6 +174,52 @@ template <class S> struct SpinorView {
       for (int k = 0; k < 12; k++) { v[k].x *= f; v[k].y *= f; }
     }
   }
+  // the neighbour in direction MU (same as load_raw; see WaveView)
+  template <int MU>
+  __device__ __forceinline__ typename S::A load_raw_dir(typename S::V (&v)[12], int j) const {
+    return load_raw(v, j);
+  }
+};
+
+// 16-bit neighbour spinors shared through the wavefront: every lane holds the
+// stored words of the input site with its own index (13 words), and a hop in x
+// or y whose neighbour index lies in the same wave takes the words from that
+// lane (ds_bpermute) instead of the vector memory path; lanes whose neighbour
+// is outside the wave load it (the load then runs for those lanes only).  The
+// y neighbours are shared only where at least half of them are in the wave
+// (share_y, uniform: the y stride in elements is at most 32).
+struct WaveView16 {
+  SpinorView<SF16> g;
+  const uint32_t (&w)[13];
+  int wave_base;
+  bool share_y;
+  __device__ __forceinline__ float load_raw(float2 (&v)[12], int j) const { return g.load_raw(v, j); }
+  __device__ __forceinline__ float load_pair_raw(float2 (&v)[6], int j, int s0) const {
+    return g.load_pair_raw(v, j, s0);
+  }
+  template <int MU>
+  __device__ __forceinline__ float load_raw_dir(float2 (&v)[12], int j) const {
+    if constexpr (MU >= 2) {
+      return g.load_raw(v, j);
+    } else {
+      if (MU == 1 && !share_y) return g.load_raw(v, j);
+      const int src = j - wave_base;
+      const bool inwave = (unsigned)src < 64u;
+      const int addr = (inwave ? src : 0) * 4;
+      uint32_t x[13];
+#pragma unroll
+      for (int k = 0; k < 13; k++) x[k] = (uint32_t)__builtin_amdgcn_ds_bpermute(addr, (int)w[k]);
+      float f;
+      if (inwave) {
+#pragma unroll
+        for (int k = 0; k < 12; k++) v[k] = SF16::raw(__builtin_bit_cast(short2, x[k]));
+        f = __builtin_bit_cast(float, x[12]) * SF16::kInv;
+      } else {
+        f = g.load_raw(v, j);
+      }
+      return f;
+    }
+  }
 };
 
 template <class S>

@@ -147,7 +147,19 @@ __global__ __launch_bounds__(kBlock) void k_dslash_multi(MultiArgs<S, G> a) {
     int xc[4];
     site_coords(g, e, site, xc[0], xc[1], xc[2], xc[3]);
     const SpinorView<S> in(a.in, vhe);
-    hop_all<S, G, DAG, NR>(acc, a.U, in, g, e, site, xc, r);
+    if constexpr (HB_WAVE_SHARE && std::is_same<S, SF16>::value) {
+      // the lanes of a wave hold consecutive (site, RHS) elements; the x and y
+      // neighbours (same RHS) come from other lanes where they are in the wave
+      uint32_t w[13];
+      const uint32_t vo = (uint32_t)tid * 4u;
+#pragma unroll
+      for (int k = 0; k < 12; k++) w[k] = __builtin_amdgcn_raw_buffer_load_b32(in.r, vo, in.plane * k, 0);
+      w[12] = __builtin_amdgcn_raw_buffer_load_b32(in.rn, vo, 0, 0);
+      const WaveView16 wv{in, w, tid - (int)(threadIdx.x & 63), g.L[0] / 2 * NR <= 32};
+      hop_all<S, G, DAG, NR, WaveView16>(acc, a.U, wv, g, e, site, xc, r);
+    } else {
+      hop_all<S, G, DAG, NR>(acc, a.U, in, g, e, site, xc, r);
+    }
   }
   const int ie = site * NR + r;
   if constexpr (EPI == EPI_CLINV) {
