@@ -770,9 +770,11 @@ int cg_multi_prec(Context *c, MultiPlan &plan, int op, int nrhs, const double *k
           }
           c->multi_tile = 256;
           HB_TRY(r);
-          // the partial last block costs as much as a full one
+          // the partial last block costs as much as a full one; the smaller
+          // workgroup must win by 3 % (a single matvec favours it slightly over
+          // what whole solves show)
           const int nblk = (nrhs + nr - 1) / nr;
-          const double total = ms * nr * nblk;
+          const double total = ms * nr * nblk * (tl < 256 ? 1.03 : 1.0);
           if (total < best) {
             best = total;
             block = nr;
