@@ -147,9 +147,11 @@ __global__ __launch_bounds__(kBlock) void k_dslash_multi(MultiArgs<S, G> a) {
     int xc[4];
     site_coords(g, e, site, xc[0], xc[1], xc[2], xc[3]);
     const SpinorView<S> in(a.in, vhe);
-    if constexpr (HB_WAVE_SHARE && std::is_same<S, SF16>::value) {
+    if constexpr (HB_WAVE_SHARE && std::is_same<S, SF16>::value && NR <= 2) {
       // the lanes of a wave hold consecutive (site, RHS) elements; the x and y
-      // neighbours (same RHS) come from other lanes where they are in the wave
+      // neighbours (same RHS) come from other lanes where they are in the wave.
+      // From NR = 4 on a wave spans 16 sites or fewer, and the extra registers of
+      // the own-index spinor cost more than the shared x hops save (measured)
       uint32_t w[13];
       const uint32_t vo = (uint32_t)tid * 4u;
 #pragma unroll
