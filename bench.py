@@ -79,8 +79,16 @@ def parse():
                     help="outer steps per trajectory; a comma list runs the trajectories once per "
                          "step count from the same start (the step-size scaling of Delta H)")
     ap.add_argument("--hmc-mscale", type=int, default=2)
+    ap.add_argument("--hmc-model", default="B", choices=["B", "C"],
+                    help="Hasenbusch time-scale split (hmc_integrator.F90): B = both pseudofermion "
+                         "forces on the outer steps; C = the rho-shifted one with the gauge force on "
+                         "the inner steps")
     ap.add_argument("--hmc-therm", type=int, default=3,
                     help="force-accepted trajectories from the smeared start before the measured ones")
+    ap.add_argument("--hmc-tile", default="", metavar="LATTICE",
+                    help="make the start on this smaller lattice (smearing, --hmc-quench, "
+                         "--hmc-therm) and repeat its links periodically to fill --hmc-lattice "
+                         "(each of its extents must divide the full one)")
     ap.add_argument("--hmc-quench", type=int, default=0, metavar="NTRAJ",
                     help="pure-gauge molecular-dynamics trajectories (tau 1, 20 steps, fresh momenta "
                          "each, no accept/reject) at --hmc-quench-beta before the dynamical "
@@ -117,6 +125,30 @@ def measured_traffic(L):
 
 def log(*a):
     print("[bench]", *a, file=sys.stderr, flush=True)
+
+
+def tile_links(us, Ls, L):
+    """Links of the lattice L (host layout [mu][eo][site][3][3][2], BQCD's
+    even-odd site index i = (x + Lx (y + Ly (z + Lz t))) / 2) built by repeating
+    those of the smaller lattice Ls periodically: U_mu(x) = U_small_mu(x mod Ls).
+    The extents are even, so a site and its image have the same parity, and
+    every plaquette of L is a copy of one of Ls."""
+    import numpy as np
+
+    if any(n % s or s % 2 for n, s in zip(L, Ls)):
+        raise ValueError("cannot tile %r with %r" % (L, Ls))
+    vh = L[0] * L[1] * L[2] * L[3] // 2
+    out = np.empty((4, 2, vh) + us.shape[3:], dtype=us.dtype)
+    x = np.arange(L[0])[None, None, :]
+    y = np.arange(L[1])[None, :, None]
+    z = np.arange(L[2])[:, None, None]
+    for t in range(L[3]):
+        j = (x + L[0] * (y + L[1] * (z + L[2] * t))).ravel()
+        eo = ((x + y + z + t) & 1).ravel()
+        xs, ys, zs, ts = x % Ls[0], y % Ls[1], z % Ls[2], t % Ls[3]
+        js = (xs + Ls[0] * (ys + Ls[1] * (zs + Ls[2] * ts))).ravel()
+        out[:, eo, j >> 1] = us[:, eo, js >> 1]
+    return out
 
 
 def cpu_baseline(g, b_dev, kappa, csw, gpu_iters, cpu_iters):
@@ -252,55 +284,76 @@ def batched(args, torch, dev, stream):
 
 def hmc(args, torch, dev, stream):
     """HMC trajectories: Wilson gauge action beta 5.29, clover csw 1.9, two
-    Hasenbusch pseudofermions (model B: S_f1 with W = M~ + rho, S_f2 the
-    ratio), Omelyan integrator, trajectory length 1 in --hmc-steps outer
-    steps with --hmc-mscale gauge steps each, Gaussian momenta and noise from
-    the device generator, Metropolis accept/reject.  The same trajectories
-    (same start, same draws) run with the mixed-precision batched solver and
-    with the FP64 solver."""
+    Hasenbusch pseudofermions (S_f1 with W = M~ + rho, S_f2 the ratio),
+    Omelyan integrator, trajectory length 1 in --hmc-steps outer steps with
+    --hmc-mscale inner steps each (model B: the gauge force inside, both
+    pseudofermion forces outside; model C: S_f1 and the clover determinant
+    move inside), Gaussian momenta and noise from the device generator,
+    Metropolis accept/reject.  The same trajectories (same start, same draws)
+    run with the mixed-precision batched solver and with the FP64 solver."""
     import bqcd_hip
 
     L = tuple(int(v) for v in args.hmc_lattice.split(","))
     beta, kappa, csw, rho = 5.29, args.hmc_kappa, args.csw, args.hmc_rho
-    g = bqcd_hip.Lattice(L)
-    g.set_stream(stream)
-    vh = g.volh
-    g.gauge_random(args.seed + 11, 0.0)
-    g.gauge_smear(args.sweeps, args.alpha)
-    p = torch.empty((4, 2, 8, vh), dtype=torch.float64, device=dev)
-    eta1 = torch.empty((12, vh, 2), dtype=torch.float64, device=dev)
-    eta2 = torch.empty_like(eta1)
-    rest = 1e-20 * 12 * vh  # |r|/|b| ~ 1e-10 for the unit-variance sources
     rng = __import__("random").Random(args.seed + 12)
-    log("hmc start: plaquette %.6f after %d smearing sweeps" % (g.plaquette(), args.sweeps))
-    if args.hmc_quench:
+
+    def draws(lat):
+        return (torch.empty((4, 2, 8, lat.volh), dtype=torch.float64, device=dev),
+                torch.empty((12, lat.volh, 2), dtype=torch.float64, device=dev),
+                torch.empty((12, lat.volh, 2), dtype=torch.float64, device=dev))
+
+    def make_start(lat):
+        """smeared Haar links, pure-gauge trajectories, force-accepted dynamical ones"""
+        lat.gauge_random(args.seed + 11, 0.0)
+        lat.gauge_smear(args.sweeps, args.alpha)
+        p, eta1, eta2 = draws(lat)
+        log("hmc start (%s): plaquette %.6f after %d smearing sweeps"
+            % ("x".join(map(str, lat.L)), lat.plaquette(), args.sweeps))
         t0 = time.perf_counter()
         dt, nstep = 0.05, 20
         for k in range(args.hmc_quench):
             # leapfrog of hmc_leap_frog.F90 with the gauge force only
-            g.gaussian_d(p, args.seed + 5000 + k, 1.0)
-            g.dsg_d(p, 0.5 * dt, args.hmc_quench_beta)
+            lat.gaussian_d(p, args.seed + 5000 + k, 1.0)
+            lat.dsg_d(p, 0.5 * dt, args.hmc_quench_beta)
             for s in range(nstep):
-                g.hmc_u_d(p, dt, 0.0)
-                g.dsg_d(p, dt if s < nstep - 1 else 0.5 * dt, args.hmc_quench_beta)
+                lat.hmc_u_d(p, dt, 0.0)
+                lat.dsg_d(p, dt if s < nstep - 1 else 0.5 * dt, args.hmc_quench_beta)
             if k % 5 == 4 or k == args.hmc_quench - 1:
                 torch.cuda.synchronize()
                 log("hmc pure-gauge trajectory %d (beta %.3f): plaquette %.6f, %.1f s"
-                    % (k, args.hmc_quench_beta, g.plaquette(), time.perf_counter() - t0))
-    # the smeared start is far from equilibrium at beta 5.29: force-accepted
-    # trajectories (as the reference's hmc_accept_first) bring it closer
-    therm = bqcd_hip.hmc_params(beta=beta, kappa=kappa, csw=csw, rho=rho, tau=0.5 / args.hmc_therm_steps,
-                                ntau=args.hmc_therm_steps, m_scale=args.hmc_mscale, model="B",
-                                integrator="omelyan", rest=rest, maxiter=50000, precision="f16",
-                                mre_vectors=args.hmc_mre, batch=True, verbose=True)
-    for k in range(args.hmc_therm):
-        seed = args.seed + 7000 + 100 * k
-        g.gaussian_d(p, seed + 1, 1.0)
-        g.gaussian_d(eta1, seed + 2, 0.5)
-        g.gaussian_d(eta2, seed + 3, 0.5)
-        out = g.hmc_trajectory_d(therm, p, eta1, eta2, accept_random=-1.0)
-        log("hmc thermalisation %d: dH %.4f plaquette %.6f, %.1f s"
-            % (k, out["h_new"] - out["h_old"], out["plaquette"], out["seconds"]))
+                    % (k, args.hmc_quench_beta, lat.plaquette(), time.perf_counter() - t0))
+        # the smeared start is far from equilibrium at beta 5.29: force-accepted
+        # trajectories (as the reference's hmc_accept_first) bring it closer
+        therm = bqcd_hip.hmc_params(beta=beta, kappa=kappa, csw=csw, rho=rho, tau=0.5 / args.hmc_therm_steps,
+                                    ntau=args.hmc_therm_steps, m_scale=args.hmc_mscale, model=args.hmc_model,
+                                    integrator="omelyan", rest=1e-20 * 12 * lat.volh, maxiter=50000,
+                                    precision="f16", mre_vectors=args.hmc_mre, batch=True, verbose=True)
+        for k in range(args.hmc_therm):
+            seed = args.seed + 7000 + 100 * k
+            lat.gaussian_d(p, seed + 1, 1.0)
+            lat.gaussian_d(eta1, seed + 2, 0.5)
+            lat.gaussian_d(eta2, seed + 3, 0.5)
+            out = lat.hmc_trajectory_d(therm, p, eta1, eta2, accept_random=-1.0)
+            log("hmc thermalisation %d: dH %.4f plaquette %.6f, %.1f s"
+                % (k, out["h_new"] - out["h_old"], out["plaquette"], out["seconds"]))
+        del p, eta1, eta2
+
+    g = bqcd_hip.Lattice(L)
+    g.set_stream(stream)
+    vh = g.volh
+    if args.hmc_tile:
+        Ls = tuple(int(v) for v in args.hmc_tile.split(","))
+        gs = bqcd_hip.Lattice(Ls)
+        gs.set_stream(stream)
+        make_start(gs)
+        g.load_gauge(tile_links(gs.save_gauge(), Ls, L))
+        gs.close()
+        log("hmc start: the links repeated %s times: plaquette %.6f"
+            % ("x".join(str(n // s) for n, s in zip(L, Ls)), g.plaquette()))
+    else:
+        make_start(g)
+    p, eta1, eta2 = draws(g)
+    rest = 1e-20 * 12 * vh  # |r|/|b| ~ 1e-10 for the unit-variance sources
     g.gauge_backup(1)
     runs = {}
     solvers = {"mixed16_batched": ("f16", True), "mixed32_batched": ("f32", True), "fp64": ("f64", False)}
@@ -312,7 +365,7 @@ def hmc(args, torch, dev, stream):
         key = name if len(steps) == 1 else "%s@%d" % (name, nsteps)
         g.gauge_restore(1)
         prm = bqcd_hip.hmc_params(beta=beta, kappa=kappa, csw=csw, rho=rho, tau=1.0 / nsteps,
-                                  ntau=nsteps, m_scale=args.hmc_mscale, model="B",
+                                  ntau=nsteps, m_scale=args.hmc_mscale, model=args.hmc_model,
                                   integrator="omelyan", rest=rest, maxiter=50000, precision=prec,
                                   mre_vectors=args.hmc_mre, batch=batch, verbose=True)
         trajs = []
@@ -344,9 +397,14 @@ def hmc(args, torch, dev, stream):
            "lattice": list(L), "beta": beta, "kappa": kappa, "csw": csw, "rho": rho,
            "integrator": "Omelyan (2MN), tau 1, %s outer steps, %d gauge steps each"
                          % (args.hmc_steps, args.hmc_mscale), "mre_vectors": args.hmc_mre,
-           "trajectories": args.hmc, "first_trajectory": args.hmc_first, "thermalisation": "%s%d force-accepted trajectories of length 0.5 in %d steps"
-                             % ("%d pure-gauge MD trajectories at beta %.3f, then " % (args.hmc_quench, args.hmc_quench_beta)
-                                if args.hmc_quench else "", args.hmc_therm, args.hmc_therm_steps), "runs": runs,
+           "model": args.hmc_model,
+           "trajectories": args.hmc, "first_trajectory": args.hmc_first,
+           "thermalisation": "%s%s%d force-accepted trajectories of length 0.5 in %d steps%s"
+                             % ("on %s: " % args.hmc_tile.replace(",", "x") if args.hmc_tile else "",
+                                "%d pure-gauge MD trajectories at beta %.3f, then " % (args.hmc_quench, args.hmc_quench_beta)
+                                if args.hmc_quench else "", args.hmc_therm, args.hmc_therm_steps,
+                                ", links then repeated periodically" if args.hmc_tile else ""),
+           "runs": runs,
            "accept_random": accept_u}
     if "fp64" in runs and "mixed16_batched" in runs:
         res["speedup_vs_fp64"] = (runs["fp64"]["s_per_trajectory"]
