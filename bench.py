@@ -70,32 +70,35 @@ def parse():
                          "same byte count (for hardware-counter runs), then exit")
     ap.add_argument("--hmc", type=int, default=0, metavar="NTRAJ",
                     help="run NTRAJ HMC trajectories of the trajectory workload (two Hasenbusch "
-                         "pseudofermions, Omelyan, tau=1, 20 outer steps) with the mixed-precision "
-                         "batched solver and with the FP64 solver, print the report, then exit")
+                         "pseudofermions, Omelyan, tau=1, 20 outer steps, from a start thermalised "
+                         "on --hmc-tile) with the mixed-precision batched solver and with the FP64 "
+                         "solver, print the report, then exit")
     ap.add_argument("--hmc-lattice", default="48,48,48,96")
     ap.add_argument("--hmc-kappa", type=float, default=0.1365)
     ap.add_argument("--hmc-rho", type=float, default=0.1)
     ap.add_argument("--hmc-steps", default="20",
                     help="outer steps per trajectory; a comma list runs the trajectories once per "
                          "step count from the same start (the step-size scaling of Delta H)")
-    ap.add_argument("--hmc-mscale", type=int, default=2)
-    ap.add_argument("--hmc-model", default="B", choices=["B", "C"],
+    ap.add_argument("--hmc-mscale", type=int, default=2, help="inner steps per outer step")
+    ap.add_argument("--hmc-model", default="C", choices=["B", "C"],
                     help="Hasenbusch time-scale split (hmc_integrator.F90): B = both pseudofermion "
                          "forces on the outer steps; C = the rho-shifted one with the gauge force on "
                          "the inner steps")
-    ap.add_argument("--hmc-therm", type=int, default=3,
-                    help="force-accepted trajectories from the smeared start before the measured ones")
-    ap.add_argument("--hmc-tile", default="", metavar="LATTICE",
+    ap.add_argument("--hmc-therm", type=int, default=60,
+                    help="force-accepted trajectories of length 0.5 from the smeared start (on the "
+                         "--hmc-tile lattice when given) before the measured ones")
+    ap.add_argument("--hmc-tile", default="16,16,16,32", metavar="LATTICE",
                     help="make the start on this smaller lattice (smearing, --hmc-quench, "
                          "--hmc-therm) and repeat its links periodically to fill --hmc-lattice "
-                         "(each of its extents must divide the full one)")
+                         "(each of its extents must divide the full one); empty: start on the "
+                         "full lattice")
     ap.add_argument("--hmc-quench", type=int, default=0, metavar="NTRAJ",
                     help="pure-gauge molecular-dynamics trajectories (tau 1, 20 steps, fresh momenta "
                          "each, no accept/reject) at --hmc-quench-beta before the dynamical "
                          "thermalisation: they give the smeared start equilibrium-like short-distance "
                          "fluctuations at the cost of gauge forces only")
     ap.add_argument("--hmc-quench-beta", type=float, default=5.7)
-    ap.add_argument("--hmc-therm-steps", type=int, default=20,
+    ap.add_argument("--hmc-therm-steps", type=int, default=10,
                     help="outer steps of each thermalisation trajectory")
     ap.add_argument("--hmc-mre", type=int, default=4,
                     help="chronological initial-guess vectors per pseudofermion (mre.F90)")
