@@ -94,10 +94,9 @@ int cg_f64(Context *c, int op, double kappa, double rho, double2 *x, const doubl
   HB_TRY(ensure_host_stage(c, sizeof(CgState)));
   CgState *hs = static_cast<CgState *>(c->host_stage);
 
-  hipEvent_t e0, e1;
-  HB_HIP(hipEventCreate(&e0));
-  HB_HIP(hipEventCreate(&e1));
-  HB_HIP(hipEventRecord(e0, c->stream));
+  EventTimer tm;
+  HB_TRY(tm.create());
+  HB_TRY(tm.start(c->stream));
 
   CgState init{};
   init.rest = (double)rest;
@@ -134,13 +133,11 @@ int cg_f64(Context *c, int op, double kappa, double rho, double2 *x, const doubl
   // true residual |b - A x|^2 in FP64 (invert_quda.F90:155-168)
   HB_TRY(apply_f64(c, op, kappa, rho, q, x, t, y, nullptr, nullptr));
   HB_TRY(residual(c, q, b, &st->res[0]));
-  HB_HIP(hipEventRecord(e1, c->stream));
+  HB_TRY(tm.stop(c->stream));
   HB_HIP(hipMemcpyAsync(hs, st, sizeof(CgState), hipMemcpyDeviceToHost, c->stream));
   HB_HIP(hipStreamSynchronize(c->stream));
   float ms = 0.f;
-  HB_HIP(hipEventElapsedTime(&ms, e0, e1));
-  HB_HIP(hipEventDestroy(e0));
-  HB_HIP(hipEventDestroy(e1));
+  HB_TRY(tm.elapsed(&ms));
   c->last_solve_ms = ms;
   c->last_n_matvec = nmv + 1;
   if (iters) *iters = niter;

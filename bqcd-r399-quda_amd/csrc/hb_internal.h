@@ -55,6 +55,38 @@ void set_error(const char *fmt, ...);
     if (r_ != HB_OK) return r_;                                                  \
   } while (0)
 
+// two timing events on one stream, destroyed on every return path (an error
+// return between start() and the end of a solve must not leak them)
+struct EventTimer {
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  EventTimer() = default;
+  EventTimer(const EventTimer &) = delete;
+  EventTimer &operator=(const EventTimer &) = delete;
+  ~EventTimer() {
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+  }
+  int create() {
+    HB_HIP(hipEventCreate(&e0));
+    HB_HIP(hipEventCreate(&e1));
+    return HB_OK;
+  }
+  int start(hipStream_t s) {
+    HB_HIP(hipEventRecord(e0, s));
+    return HB_OK;
+  }
+  int stop(hipStream_t s) {
+    HB_HIP(hipEventRecord(e1, s));
+    return HB_OK;
+  }
+  // waits for the stop event, then the time between the two in ms
+  int elapsed(float *ms) {
+    HB_HIP(hipEventSynchronize(e1));
+    HB_HIP(hipEventElapsedTime(ms, e0, e1));
+    return HB_OK;
+  }
+};
+
 // ----------------------------------------------------------------------------
 // geometry passed by value to every kernel
 // ----------------------------------------------------------------------------

@@ -348,10 +348,9 @@ int cg_mixed_impl(Context *c, int op, double kappa, double rho, double2 *x, cons
   MixState *hs = static_cast<MixState *>(c->host_stage);
   HB_TRY(ensure_lowprec(c, G::kBytes == 8 ? 32 : 16));
 
-  hipEvent_t e0, e1;
-  HB_HIP(hipEventCreate(&e0));
-  HB_HIP(hipEventCreate(&e1));
-  HB_HIP(hipEventRecord(e0, c->stream));
+  EventTimer tm;
+  HB_TRY(tm.create());
+  HB_TRY(tm.start(c->stream));
   MixState init{};
   init.rest = (double)rest;
   init.delta = delta;
@@ -404,12 +403,9 @@ int cg_mixed_impl(Context *c, int op, double kappa, double rho, double2 *x, cons
       chunk *= 2;
     }
   }
-  HB_HIP(hipEventRecord(e1, c->stream));
-  HB_HIP(hipStreamSynchronize(c->stream));
+  HB_TRY(tm.stop(c->stream));
   float ms = 0.f;
-  HB_HIP(hipEventElapsedTime(&ms, e0, e1));
-  HB_HIP(hipEventDestroy(e0));
-  HB_HIP(hipEventDestroy(e1));
+  HB_TRY(tm.elapsed(&ms));
   c->last_solve_ms = ms;
   c->last_n_matvec = nmv + hs->iter;
   if (iters) *iters = hs->iter;
@@ -462,19 +458,13 @@ int time_dslash_lp(Context *c, int reps, const double2 *in64, double *ms_out) {
   a.C = clov ? LP<S>::clov(c, HB_CLOVER_AINV) : nullptr;
   a.Cn = clov ? LP<S>::norm(c, HB_CLOVER_AINV) : nullptr;
   for (int k = 0; k < 2; k++) HB_TRY(launch_dslash(a, false, EPI_CLINV, c->stream));
-  hipEvent_t e0, e1;
-  HB_HIP(hipEventCreate(&e0));
-  HB_HIP(hipEventCreate(&e1));
-  HB_HIP(hipEventRecord(e0, c->stream));
-  int r = HB_OK;
-  for (int k = 0; k < reps && r == HB_OK; k++) r = launch_dslash(a, false, EPI_CLINV, c->stream);
-  HB_HIP(hipEventRecord(e1, c->stream));
-  HB_HIP(hipEventSynchronize(e1));
+  EventTimer tm;
+  HB_TRY(tm.create());
+  HB_TRY(tm.start(c->stream));
+  for (int k = 0; k < reps; k++) HB_TRY(launch_dslash(a, false, EPI_CLINV, c->stream));
+  HB_TRY(tm.stop(c->stream));
   float ms = 0.f;
-  HB_HIP(hipEventElapsedTime(&ms, e0, e1));
-  (void)hipEventDestroy(e0);
-  (void)hipEventDestroy(e1);
-  HB_TRY(r);
+  HB_TRY(tm.elapsed(&ms));
   *ms_out = ms / reps;
   return HB_OK;
 }

@@ -628,21 +628,28 @@ int cg_multi_block(Context *c, const MultiPlan &plan, const MultiFields<S, G> &f
     return HB_OK;
   };
   const int chunk = 8;
-  hipGraph_t graph = nullptr;
-  hipGraphExec_t exec = nullptr;
+  // the captured chunk and its executable, destroyed on every return path
+  struct Graph {
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    ~Graph() {
+      if (exec) (void)hipGraphExecDestroy(exec);
+      if (graph) (void)hipGraphDestroy(graph);
+    }
+  } gr;
   HB_HIP(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
   int crc = HB_OK;
   for (int k = 0; k < chunk && crc == HB_OK; k++) crc = iteration();
-  hipError_t ce = hipStreamEndCapture(c->stream, &graph);
+  hipError_t ce = hipStreamEndCapture(c->stream, &gr.graph);
   HB_TRY(crc);
   if (ce != hipSuccess) {
     set_error("hipStreamEndCapture: %s", hipGetErrorString(ce));
     return HB_E_HIP;
   }
-  HB_HIP(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+  HB_HIP(hipGraphInstantiate(&gr.exec, gr.graph, nullptr, nullptr, 0));
   int result = HB_OK;
   for (;;) {
-    hipError_t le = hipGraphLaunch(exec, c->stream);
+    hipError_t le = hipGraphLaunch(gr.exec, c->stream);
     if (le == hipSuccess) le = hipMemcpyAsync(hs, st, sizeof(MultiState), hipMemcpyDeviceToHost, c->stream);
     if (le == hipSuccess) le = hipStreamSynchronize(c->stream);
     if (le != hipSuccess) {
@@ -675,8 +682,6 @@ int cg_multi_block(Context *c, const MultiPlan &plan, const MultiFields<S, G> &f
     }
     if (!any_active) break;
   }
-  (void)hipGraphExecDestroy(exec);
-  (void)hipGraphDestroy(graph);
   HB_TRY(result);
   for (int k = 0; k < nb; k++) {
     if (iters) iters[idx[k]] = hs->iter[k];
@@ -716,20 +721,16 @@ int time_block(Context *c, const MultiFields<S, G> &f, const MultiPlan &plan, do
   double kap[16], rh[16] = {0};
   for (int k = 0; k < 16; k++) kap[k] = 0.13;
   for (int k = 0; k < NR; k++) grp[k] = k % std::max(plan.ngroups, 1);
-  hipEvent_t e0, e1;
-  HB_HIP(hipEventCreate(&e0));
-  HB_HIP(hipEventCreate(&e1));
+  EventTimer tm;
+  HB_TRY(tm.create());
   HB_TRY((multi_apply<S, G, NR>(c, f, plan.ngroups, grp, kap, rh, q, p, t, y, st, plan.clover)));
-  HB_HIP(hipEventRecord(e0, c->stream));
+  HB_TRY(tm.start(c->stream));
   const int reps = 3;
   for (int k = 0; k < reps; k++)
     HB_TRY((multi_apply<S, G, NR>(c, f, plan.ngroups, grp, kap, rh, q, p, t, y, st, plan.clover)));
-  HB_HIP(hipEventRecord(e1, c->stream));
-  HB_HIP(hipEventSynchronize(e1));
+  HB_TRY(tm.stop(c->stream));
   float ms = 0.f;
-  HB_HIP(hipEventElapsedTime(&ms, e0, e1));
-  HB_HIP(hipEventDestroy(e0));
-  HB_HIP(hipEventDestroy(e1));
+  HB_TRY(tm.elapsed(&ms));
   *ms_per_rhs = ms / reps / NR;
   return HB_OK;
 }
