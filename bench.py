@@ -285,6 +285,17 @@ def batched(args, torch, dev, stream):
     return res
 
 
+def hmc_lattices(args):
+    """the trajectory lattice and the lattice its start is made on (None: the same)"""
+    L = tuple(int(v) for v in args.hmc_lattice.split(","))
+    Ls = tuple(int(v) for v in args.hmc_tile.split(",")) if args.hmc_tile else None
+    if Ls and (len(Ls) != 4 or any(n % s or s % 2 for n, s in zip(L, Ls))):
+        raise SystemExit("--hmc-tile %s does not divide --hmc-lattice %s into even extents: give "
+                         "one that does, or --hmc-tile '' to start on the full lattice"
+                         % (args.hmc_tile, args.hmc_lattice))
+    return L, Ls
+
+
 def hmc(args, torch, dev, stream):
     """HMC trajectories: Wilson gauge action beta 5.29, clover csw 1.9, two
     Hasenbusch pseudofermions (S_f1 with W = M~ + rho, S_f2 the ratio),
@@ -296,7 +307,7 @@ def hmc(args, torch, dev, stream):
     run with the mixed-precision batched solver and with the FP64 solver."""
     import bqcd_hip
 
-    L = tuple(int(v) for v in args.hmc_lattice.split(","))
+    L, Ls = hmc_lattices(args)
     beta, kappa, csw, rho = 5.29, args.hmc_kappa, args.csw, args.hmc_rho
     rng = __import__("random").Random(args.seed + 12)
 
@@ -344,8 +355,7 @@ def hmc(args, torch, dev, stream):
     g = bqcd_hip.Lattice(L)
     g.set_stream(stream)
     vh = g.volh
-    if args.hmc_tile:
-        Ls = tuple(int(v) for v in args.hmc_tile.split(","))
+    if Ls:
         gs = bqcd_hip.Lattice(Ls)
         gs.set_stream(stream)
         make_start(gs)
@@ -446,6 +456,7 @@ def main():
     import bqcd_hip
 
     if args.hmc:
+        hmc_lattices(args)  # argument errors before any GPU work
         dev = torch.device("cuda:0")
         torch.cuda.set_device(dev)
         stream = torch.cuda.Stream(device=dev)

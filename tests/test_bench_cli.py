@@ -42,3 +42,11 @@ def test_tile_links_repeats_every_plaquette():
     assert np.array_equal(ub[:, eo, jb // 2], us[:, eo, js // 2])
     with pytest.raises(ValueError):
         bench.tile_links(us, Ls, (8, 4, 8, 8))   # 6 does not divide 8
+
+
+def test_hmc_tile_that_does_not_divide_is_refused():
+    p = subprocess.run([sys.executable, os.path.join(REPO, "bench.py"), "--hmc", "1",
+                        "--hmc-lattice", "24,24,24,48"], capture_output=True, text=True, timeout=120)
+    assert p.returncode == 1
+    assert "does not divide" in p.stderr
+    assert p.stdout == ""
