@@ -408,7 +408,7 @@ def hmc(args, torch, dev, stream):
     res = {"workload": "HMC trajectories, clover-Wilson Nf=2 with two Hasenbusch pseudofermions, %s"
                        % "x".join(map(str, L)),
            "lattice": list(L), "beta": beta, "kappa": kappa, "csw": csw, "rho": rho,
-           "integrator": "Omelyan (2MN), tau 1, %s outer steps, %d gauge steps each"
+           "integrator": "Omelyan (2MN), tau 1, %s outer steps, %d inner steps each"
                          % (args.hmc_steps, args.hmc_mscale), "mre_vectors": args.hmc_mre,
            "model": args.hmc_model,
            "trajectories": args.hmc, "first_trajectory": args.hmc_first,
