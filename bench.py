@@ -34,6 +34,8 @@ DSLASH_BYTES_PER_SITE = 8 * 144 + 192 + 192
 # operator): 8 two-row links of 24 B, the input and output spinors (48 B + 4 B
 # norm each), two clover blocks of 18 words (72 B) plus their (m, c) float2
 DSLASH16_CLINV_BYTES_PER_SITE = 8 * 24 + 2 * (48 + 4) + 2 * (72 + 8)
+# the usual hopping-term count per output site: 8 spin projections, 8 SU(3) x
+# half-spinor products, reconstruction and the sum over directions
 DSLASH_FLOPS_PER_SITE = 1320
 HBM_PEAK_GBS = 8000.0
 
@@ -603,6 +605,7 @@ def main():
                   "kernel": "k_dslash<SF16, SF16, false, EPI_CLINV> (16-bit hopping matrix with "
                             "the A^-1 epilogue; 4 such launches per inner iteration)",
                   "bytes_per_launch": b16, "ms_per_launch": ms16,
+                  "GFlops_hopping": DSLASH_FLOPS_PER_SITE * vh / (ms16 * 1e-3) / 1e9,
                   "note": "bound by the vector memory path, not HBM "
                           "(profiles/dslash_counters_32x32x32x64.md)"}
 
@@ -677,6 +680,7 @@ def main():
         "fp64_cg_algorithmic_GBs": cg_bytes / (fp64_ms * 1e-3) / 1e9,
         "dslash_fp64_ms": ds_ms,
         "dslash_fp64_GBs": ds_gbs,
+        "dslash_fp64_GFlops": DSLASH_FLOPS_PER_SITE * vh / (ds_ms * 1e-3) / 1e9,
         "roofline": {
             "bound": "hbm",
             "achieved": ds_gbs,
