@@ -115,6 +115,8 @@ def lib():
                                         ctypes.c_double, ctypes.POINTER(HmcOut)]
         L.hb_hmc_trajectory_d.argtypes = [_vp, ctypes.POINTER(HmcParams), _vp, _vp, _vp, ctypes.c_double,
                                           ctypes.POINTER(HmcOut)]
+        L.hb_hmc_test_d.argtypes = [_vp, ctypes.POINTER(HmcParams), ctypes.c_int, _vp, _vp, _vp,
+                                    ctypes.POINTER(HmcOut)]
         L.hb_gaussian_d.argtypes = [_vp, ctypes.c_ulonglong, ctypes.c_double, _vp, ctypes.c_longlong]
         L.hb_conf_write.argtypes = [ctypes.c_int * 4, ctypes.c_int * 4, _dp, ctypes.c_int, ctypes.c_char_p,
                                     ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double,
@@ -494,6 +496,18 @@ class Lattice:
         _check(lib().hb_hmc_trajectory_d(self._h, ctypes.byref(prm), _tp(p), _tp(eta1),
                                          None if eta2 is None else _tp(eta2), accept_random,
                                          ctypes.byref(out)))
+        return out.as_dict()
+
+    def hmc_test_d(self, prm, direction, p, eta1=None, eta2=None):
+        """hmc()'s reversibility-test modes (hmc.F90:124-210, hmc_test.F90:28-50):
+        direction +1 runs a force-accepted trajectory from the momenta p and the
+        noise eta1/eta2; direction -1 runs one with the step negated from the
+        momenta p and the pseudofermion fields of the trajectory before it.  Both
+        overwrite the device tensor p with the final momenta."""
+        out = HmcOut()
+        _check(lib().hb_hmc_test_d(self._h, ctypes.byref(prm), direction, _tp(p),
+                                   None if eta1 is None else _tp(eta1), None if eta2 is None else _tp(eta2),
+                                   ctypes.byref(out)))
         return out.as_dict()
 
     def gaussian_d(self, t, seed, var):

@@ -484,12 +484,21 @@ struct Traj {
   }
 };
 
+// hmc.F90:28-215.  dir = 0: a trajectory; dir = +1 / -1: the reversibility test
+// modes HMC_TEST_FORWARDS / HMC_TEST_BACKWARDS (hmc.F90:124-210): the backward
+// one negates the step and skips the draws and H before, integrating with the
+// phi fields left by the preceding trajectory.  p_out (may be null) receives
+// the final momenta.
 int trajectory(Context *c, const hb_hmc_params *prm, const double *p_d, const double *eta1_d,
-               const double *eta2_d, double accept_random, hb_hmc_out *out) {
-  HB_CHECK(prm && out && p_d && eta1_d, HB_E_ARG, "hmc: null argument");
+               const double *eta2_d, double accept_random, hb_hmc_out *out, int dir = 0,
+               double *p_out = nullptr) {
+  const bool backward = dir < 0;
+  HB_CHECK(prm && out && p_d && (eta1_d || backward), HB_E_ARG, "hmc: null argument");
   HB_CHECK(prm->model == 'A' || prm->model == 'B' || prm->model == 'C', HB_E_ARG, "hmc: model '%c'",
            prm->model);
-  HB_CHECK(prm->model == 'A' || eta2_d, HB_E_ARG, "hmc: models B and C need eta2");
+  HB_CHECK(prm->model == 'A' || eta2_d || backward, HB_E_ARG, "hmc: models B and C need eta2");
+  HB_CHECK(!backward || c->hmc_phi_model == prm->model, HB_E_STATE,
+           "hmc: a backward test trajectory needs a trajectory of model %c before it", prm->model);
   HB_CHECK(prm->ntau >= 1 && prm->m_scale >= 1, HB_E_ARG, "hmc: ntau, m_scale must be >= 1");
   HB_CHECK(prm->integrator == HB_INT_LEAPFROG || prm->integrator == HB_INT_OMELYAN, HB_E_ARG,
            "hmc: integrator %d", prm->integrator);
@@ -506,6 +515,7 @@ int trajectory(Context *c, const hb_hmc_params *prm, const double *p_d, const do
   Traj T{};
   T.c = c;
   T.prm = *prm;
+  if (backward) T.prm.tau = -prm->tau;  // hmc_test.F90:45
   T.t0 = t0;
   if (T.prm.delta <= 0.0) T.prm.delta = 0.1;
   if (T.prm.max_inner <= 0) T.prm.max_inner = 1000;
@@ -539,26 +549,29 @@ int trajectory(Context *c, const hb_hmc_params *prm, const double *p_d, const do
   if (T.clover) HB_TRY(build_clover_dev(c, T.ck));
   HB_HIP(hipMemcpyAsync(T.P, p_d, sizeof(double) * 64 * (size_t)vh, hipMemcpyDeviceToDevice, c->stream));
 
-  // hmc_init_phi.F90:28-79 (no boundary flip: the kernels apply the sign)
-  const double2 *eta1 = reinterpret_cast<const double2 *>(eta1_d);
-  double sf1_old = 0.0, sf2_old = 0.0;
-  HB_TRY(T.v.norm2(eta1, &sf1_old));
-  HB_TRY(apply_f64(c, HB_OP_W_DAG, prm->kappa, prm->rho, T.phi1, eta1, t, nullptr, nullptr, nullptr));
-  if (T.hasenbusch) {
-    const double2 *eta2 = reinterpret_cast<const double2 *>(eta2_d);
-    HB_TRY(T.v.norm2(eta2, &sf2_old));
-    HB_TRY(apply_f64(c, HB_OP_MTIL_DAG, prm->kappa, 0.0, T.phi2, eta2, t, nullptr, nullptr, nullptr));
-    HB_TRY(copy_f64(c, T.a, eta2, T.v.n));  // the noise is the initial guess (hmc_init_phi.F90:64-70)
-    HB_TRY(T.solve(HB_OP_WDAGW, prm->rho, T.a, T.phi2));
-    HB_TRY(apply_f64(c, HB_OP_W, prm->kappa, prm->rho, T.phi2, T.a, t, nullptr, nullptr, nullptr));
-  }
+  double sf1_old = 0.0, sf2_old = 0.0, sd_old = 0.0, sg_old = 0.0, sp_old = 0.0, h_old = 0.0;
+  if (!backward) {
+    // hmc_init_phi.F90:28-79 (no boundary flip: the kernels apply the sign)
+    c->hmc_phi_model = 0;
+    const double2 *eta1 = reinterpret_cast<const double2 *>(eta1_d);
+    HB_TRY(T.v.norm2(eta1, &sf1_old));
+    HB_TRY(apply_f64(c, HB_OP_W_DAG, prm->kappa, prm->rho, T.phi1, eta1, t, nullptr, nullptr, nullptr));
+    if (T.hasenbusch) {
+      const double2 *eta2 = reinterpret_cast<const double2 *>(eta2_d);
+      HB_TRY(T.v.norm2(eta2, &sf2_old));
+      HB_TRY(apply_f64(c, HB_OP_MTIL_DAG, prm->kappa, 0.0, T.phi2, eta2, t, nullptr, nullptr, nullptr));
+      HB_TRY(copy_f64(c, T.a, eta2, T.v.n));  // the noise is the initial guess (hmc_init_phi.F90:64-70)
+      HB_TRY(T.solve(HB_OP_WDAGW, prm->rho, T.a, T.phi2));
+      HB_TRY(apply_f64(c, HB_OP_W, prm->kappa, prm->rho, T.phi2, T.a, t, nullptr, nullptr, nullptr));
+    }
+    c->hmc_phi_model = prm->model;
 
-  // H before (hmc.F90:127-134)
-  double sd_old, sg_old, sp_old;
-  HB_TRY(T.sd(&sd_old));
-  HB_TRY(T.sg(&sg_old));
-  HB_TRY(T.sp(&sp_old));
-  const double h_old = sd_old + sp_old + prm->beta * sg_old + sf1_old + sf2_old;
+    // H before (hmc.F90:127-134)
+    HB_TRY(T.sd(&sd_old));
+    HB_TRY(T.sg(&sg_old));
+    HB_TRY(T.sp(&sp_old));
+    h_old = sd_old + sp_old + prm->beta * sg_old + sf1_old + sf2_old;
+  }
 
   HB_TRY(prm->integrator == HB_INT_LEAPFROG ? T.leapfrog() : T.omelyan());
 
@@ -572,7 +585,7 @@ int trajectory(Context *c, const hb_hmc_params *prm, const double *p_d, const do
   hb_hmc_out o{};
   o.h_old = h_old;
   o.h_new = h_new;
-  o.exp_dh = std::exp(h_old - h_new);
+  o.exp_dh = backward ? 0.0 : std::exp(h_old - h_new);
   o.sg_old = sg_old;
   o.sg_new = sg_new;
   o.sp_old = sp_old;
@@ -597,6 +610,8 @@ int trajectory(Context *c, const hb_hmc_params *prm, const double *p_d, const do
   o.cg_niter_max = T.niter_max;
   o.n_force_ir = T.n_ir;
   o.n_force_uv = T.n_uv;
+  if (p_out)
+    HB_HIP(hipMemcpyAsync(p_out, T.P, sizeof(double) * 64 * (size_t)vh, hipMemcpyDeviceToDevice, c->stream));
   HB_HIP(hipStreamSynchronize(c->stream));
   o.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
   *out = o;
@@ -616,6 +631,16 @@ int hb_hmc_trajectory_d(hb_context *ctx, const hb_hmc_params *prm, const double 
   HB_TRY(check_ready(c));
   HB_HIP(hipSetDevice(c->device));
   return trajectory(c, prm, p_d, eta1_d, eta2_d, accept_random, out);
+}
+
+int hb_hmc_test_d(hb_context *ctx, const hb_hmc_params *prm, int direction, double *p_d,
+                  const double *eta1_d, const double *eta2_d, hb_hmc_out *out) {
+  Context *c = reinterpret_cast<Context *>(ctx);
+  HB_TRY(check_ready(c));
+  HB_CHECK(direction == 1 || direction == -1, HB_E_ARG, "hb_hmc_test_d: direction %d (+1 or -1)", direction);
+  HB_HIP(hipSetDevice(c->device));
+  // hmc_test.F90:41-47: force-accepted in both directions
+  return trajectory(c, prm, p_d, eta1_d, eta2_d, -1.0, out, direction, p_d);
 }
 
 int hb_hmc_trajectory(hb_context *ctx, const hb_hmc_params *prm, const double *p, int site_stride,

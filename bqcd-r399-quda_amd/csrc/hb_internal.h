@@ -159,6 +159,9 @@ struct Context {
   int mre_nvec = 0;
   int mre_rank[2] = {0, 0};
   int mre_slot[2][16] = {};
+  // model of the trajectory whose pseudofermion fields phi are in the HMC's
+  // scratch slots (0: none); a backward test trajectory integrates with them
+  char hmc_phi_model = 0;
 };
 
 int ensure_dev_stage(Context *c, size_t bytes);

@@ -252,6 +252,18 @@ int hb_hmc_trajectory(hb_context *ctx, const hb_hmc_params *prm, const double *p
 int hb_hmc_trajectory_d(hb_context *ctx, const hb_hmc_params *prm, const double *p_d,
                         const double *eta1_d, const double *eta2_d, double accept_random,
                         hb_hmc_out *out);
+/* hmc() in its reversibility-test modes (hmc.F90:124-210, as hmc_test.F90:28-50
+ * drives them), on device fields.  direction = +1 (HMC_TEST_FORWARDS): a
+ * force-accepted trajectory from the momenta p_d and the noise eta*_d.
+ * direction = -1 (HMC_TEST_BACKWARDS): a force-accepted trajectory with the
+ * step negated, from the momenta p_d and the pseudofermion fields phi of the
+ * trajectory run before it on this context (same model); no noise is read
+ * (eta*_d may be NULL), and h_old, exp_dh and the _old terms of *out are 0, as
+ * hmc() leaves them.  Both directions write the final momenta back to p_d.
+ * Forward then backward returns the links and momenta to the start, and the
+ * backward h_new to the forward h_old, up to the solver tolerance. */
+int hb_hmc_test_d(hb_context *ctx, const hb_hmc_params *prm, int direction, double *p_d,
+                  const double *eta1_d, const double *eta2_d, hb_hmc_out *out);
 /* ---- BQCD configuration files (host only, no device needed) -----------
  * conf_write / conf_read (conf.F90:305-470) with field_io (comm/field_io_mpi.F90)
  * and the ConfInfoHeader / CheckSum blocks (conf_info.F90): one big-endian file

@@ -15,7 +15,9 @@ Bars: exp(-Delta H) to 1e-9 relative, final plaquette action and links to
 
 Then properties no reference pins: Omelyan and leapfrog Delta H fall as
 step^2; batched (multi-RHS) and sequential mixed-precision model-B
-trajectories agree; rejection restores the start links."""
+trajectories agree; rejection restores the start links; the reversibility
+test of hmc_test.F90 (forward, then backward with the step negated) returns
+links, momenta and H to the start."""
 import ast
 
 import numpy as np
@@ -128,4 +130,50 @@ def test_rejection_restores_links(hip):
     out = g.hmc_trajectory(prm, *draws, accept_random=2.0e300)
     assert out["accepted"] == 0
     assert np.array_equal(g.save_gauge(), u0)
+    g.close()
+
+
+@pytest.mark.parametrize("model,integrator", [("B", "leapfrog"), ("C", "omelyan")])
+def test_forward_backward_returns_to_start(hip, model, integrator):
+    """hmc_test.F90:28-50: a force-accepted trajectory, then one with the step
+    negated from its final momenta and the same pseudofermion fields.  The
+    integrators are symmetric, so the links and momenta return to the start and
+    H to its value there, up to the solver tolerance (|r|^2 <= 1e-24, no
+    chronological guesses).  A backward call with no trajectory before it is an
+    error."""
+    import torch
+
+    L = (4, 4, 4, 6)
+    lat = ol.Lattice(L)
+    u0 = lat.gauge_exp_random(9, 0.2)
+    g = hip.Lattice(L)
+    g.load_gauge(u0)
+    vh = g.volh
+    dev = torch.device("cuda:0")
+    p = torch.empty((4, 2, 8, vh), dtype=torch.float64, device=dev)
+    eta1 = torch.empty((12, vh, 2), dtype=torch.float64, device=dev)
+    eta2 = torch.empty_like(eta1)
+    g.gaussian_d(p, 101, 1.0)
+    g.gaussian_d(eta1, 102, 0.5)
+    g.gaussian_d(eta2, 103, 0.5)
+    torch.cuda.synchronize()
+    p0 = p.clone()
+    prm = hip.hmc_params(beta=5.29, kappa=0.13, csw=1.9, rho=0.1, tau=0.1, ntau=5, m_scale=2, model=model,
+                         integrator=integrator, rest=1e-24, maxiter=5000, mre_vectors=0)
+    print("reversibility", model, integrator, flush=True)
+    with pytest.raises(hip.HipError, match="backward"):
+        g.hmc_test_d(prm, -1, p)
+    fwd = g.hmc_test_d(prm, +1, p, eta1, eta2)
+    u1 = g.save_gauge()
+    bwd = g.hmc_test_d(prm, -1, p)
+    u2 = g.save_gauge()
+    torch.cuda.synchronize()
+    dp = float((p - p0).abs().max())
+    print(fwd["h_old"], fwd["h_new"], bwd["h_new"], np.max(np.abs(u2 - u0)), dp, flush=True)
+    assert fwd["accepted"] == 1 and bwd["accepted"] == 1
+    assert np.max(np.abs(u1 - u0)) > 1e-3   # the forward trajectory moved the links
+    assert np.max(np.abs(u2 - u0)) < 1e-10
+    assert dp < 1e-9
+    assert abs(bwd["h_new"] - fwd["h_old"]) < 1e-9 * abs(fwd["h_old"])
+    assert bwd["h_old"] == 0.0 and bwd["exp_dh"] == 0.0
     g.close()
