@@ -173,8 +173,16 @@ def _p(a):
     return a.ctypes.data_as(_dp)
 
 
-def _tp(t):
-    """device pointer of a torch tensor"""
+def _tp(t, numel=0):
+    """device pointer of a contiguous float64 GPU tensor holding at least numel
+    values: the library reads and writes that many through it, so anything
+    else is refused here rather than reaching a kernel"""
+    if not (getattr(t, "is_cuda", False) and str(t.dtype) == "torch.float64" and t.is_contiguous()):
+        raise HipError("expected a contiguous float64 GPU tensor, got %s of shape %s on %s"
+                       % (getattr(t, "dtype", type(t).__name__), tuple(getattr(t, "shape", ())),
+                          getattr(t, "device", "the host")))
+    if t.numel() < numel:
+        raise HipError("device tensor of %d values where %d are needed" % (t.numel(), numel))
     return ctypes.c_void_p(t.data_ptr())
 
 
@@ -221,6 +229,15 @@ class Lattice:
             self.close()
         except Exception:
             pass
+
+    # ---- device tensors handed to the library ---------------------------------
+    def _s(self, t):
+        """one parity's spinor field: (12, volh, 2) float64 on the GPU"""
+        return _tp(t, 24 * self.volh)
+
+    def _m(self, t):
+        """momenta: (4, 2, 8, volh) float64 on the GPU"""
+        return _tp(t, 64 * self.volh)
 
     # ---- host allocation helpers ------------------------------------------
     def spinor(self):
@@ -366,7 +383,7 @@ class Lattice:
         """average ms of the reduced-precision hopping kernel (A^-1 D, odd output) of
         the mixed solver on the even device spinor inp (12, volh, 2)"""
         ms = ctypes.c_double(0.0)
-        _check(lib().hb_time_dslash_lp(self._h, PREC[precision], reps, _tp(inp), ctypes.byref(ms)))
+        _check(lib().hb_time_dslash_lp(self._h, PREC[precision], reps, self._s(inp), ctypes.byref(ms)))
         return ms.value
 
     def multi_tile(self):
@@ -422,32 +439,32 @@ class Lattice:
         _check(lib().hb_set_stream(self._h, ctypes.c_void_p(stream.cuda_stream) if stream else None))
 
     def to_device(self, host_spinor, out):
-        _check(lib().hb_spinor_to_device(self._h, _tp(out),
+        _check(lib().hb_spinor_to_device(self._h, self._s(out),
                                          _p(np.ascontiguousarray(host_spinor, dtype=np.float64))))
         return out
 
     def to_host(self, t):
         h = self.spinor()
-        _check(lib().hb_spinor_to_host(self._h, _p(h), _tp(t)))
+        _check(lib().hb_spinor_to_host(self._h, _p(h), self._s(t)))
         return h
 
     def dslash_d(self, e, dagger, out, inp):
-        _check(lib().hb_dslash_d(self._h, e, 1 if dagger else 0, _tp(out), _tp(inp)))
+        _check(lib().hb_dslash_d(self._h, e, 1 if dagger else 0, self._s(out), self._s(inp)))
 
     def apply_d(self, op, kappa, rho, out, inp):
-        _check(lib().hb_apply_d(self._h, OP[op], kappa, rho, _tp(out), _tp(inp)))
+        _check(lib().hb_apply_d(self._h, OP[op], kappa, rho, self._s(out), self._s(inp)))
 
     def cg_d(self, op, x, b, kappa, rho=0.0, rest=1e-20, maxiter=1000, log=0):
         it = ctypes.c_int(0)
         tr = ctypes.c_double(0.0)
-        _check(lib().hb_cg_d(self._h, OP[op], kappa, rho, _tp(x), _tp(b), ctypes.c_float(rest),
+        _check(lib().hb_cg_d(self._h, OP[op], kappa, rho, self._s(x), self._s(b), ctypes.c_float(rest),
                              maxiter, log, ctypes.byref(it), ctypes.byref(tr)))
         return it.value, tr.value
 
     def cg_mixed_d(self, op, x, b, kappa, rho=0.0, rest=1e-20, maxiter=10000, log=0,
                    precision="f16", delta=0.1, max_inner=1000):
         it, nrel, tr = ctypes.c_int(0), ctypes.c_int(0), ctypes.c_double(0.0)
-        _check(lib().hb_cg_mixed_d(self._h, OP[op], kappa, rho, _tp(x), _tp(b), ctypes.c_float(rest),
+        _check(lib().hb_cg_mixed_d(self._h, OP[op], kappa, rho, self._s(x), self._s(b), ctypes.c_float(rest),
                                    maxiter, log, PREC[precision], delta, max_inner, ctypes.byref(it),
                                    ctypes.byref(nrel), ctypes.byref(tr)))
         return it.value, nrel.value, tr.value
@@ -461,24 +478,24 @@ class Lattice:
         tr = (ctypes.c_double * n)()
         _check(lib().hb_cg_multi_d(self._h, OP[op], n, (ctypes.c_double * n)(*kappas),
                                    (ctypes.c_double * n)(*rhos), csw,
-                                   (_vp * n)(*[t.data_ptr() for t in xs]),
-                                   (_vp * n)(*[t.data_ptr() for t in bs]),
+                                   (_vp * n)(*[self._s(t).value for t in xs]),
+                                   (_vp * n)(*[self._s(t).value for t in bs]),
                                    (ctypes.c_float * n)(*rests), maxiter, log, PREC[precision],
                                    block, it, tr))
         return list(it), list(tr), lib().hb_multi_block_size(self._h)
 
     def dsf_d(self, p, a, b, step, kappa, csw_kappa):
         """p (4,2,8,volh) float64 device tensor accumulates the fermion force"""
-        _check(lib().hb_dsf_d(self._h, _tp(p), step, kappa, csw_kappa, _tp(a), _tp(b)))
+        _check(lib().hb_dsf_d(self._h, self._m(p), step, kappa, csw_kappa, self._s(a), self._s(b)))
 
     def dsd_d(self, p, step, csw_kappa):
-        _check(lib().hb_dsd_d(self._h, _tp(p), step, csw_kappa))
+        _check(lib().hb_dsd_d(self._h, self._m(p), step, csw_kappa))
 
     def dsg_d(self, p, step, beta):
-        _check(lib().hb_dsg_d(self._h, _tp(p), step, beta))
+        _check(lib().hb_dsg_d(self._h, self._m(p), step, beta))
 
     def hmc_u_d(self, p, step, csw_kappa=0.0):
-        _check(lib().hb_hmc_u_d(self._h, _tp(p), step, csw_kappa))
+        _check(lib().hb_hmc_u_d(self._h, self._m(p), step, csw_kappa))
 
     # ---- HMC trajectory -------------------------------------------------------
     def hmc_trajectory(self, prm, p, eta1, eta2=None, accept_random=-1.0):
@@ -493,8 +510,8 @@ class Lattice:
 
     def hmc_trajectory_d(self, prm, p, eta1, eta2=None, accept_random=-1.0):
         out = HmcOut()
-        _check(lib().hb_hmc_trajectory_d(self._h, ctypes.byref(prm), _tp(p), _tp(eta1),
-                                         None if eta2 is None else _tp(eta2), accept_random,
+        _check(lib().hb_hmc_trajectory_d(self._h, ctypes.byref(prm), self._m(p), self._s(eta1),
+                                         None if eta2 is None else self._s(eta2), accept_random,
                                          ctypes.byref(out)))
         return out.as_dict()
 
@@ -505,8 +522,9 @@ class Lattice:
         momenta p and the pseudofermion fields of the trajectory before it.  Both
         overwrite the device tensor p with the final momenta."""
         out = HmcOut()
-        _check(lib().hb_hmc_test_d(self._h, ctypes.byref(prm), direction, _tp(p),
-                                   None if eta1 is None else _tp(eta1), None if eta2 is None else _tp(eta2),
+        _check(lib().hb_hmc_test_d(self._h, ctypes.byref(prm), direction, self._m(p),
+                                   None if eta1 is None else self._s(eta1),
+                                   None if eta2 is None else self._s(eta2),
                                    ctypes.byref(out)))
         return out.as_dict()
 

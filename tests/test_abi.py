@@ -57,3 +57,40 @@ def test_harness_fails_loudly_without_library(monkeypatch):
     monkeypatch.setattr(bqcd_hip, "_lib", None, raising=False)
     with pytest.raises(bqcd_hip.HipError):
         bqcd_hip.lib()
+
+
+def test_device_tensors_are_checked_before_they_reach_the_library():
+    """the harness hands the library only contiguous float64 GPU tensors with
+    enough values; anything else is an error on the host, never a kernel
+    reading past a buffer"""
+    import numpy as np
+    import torch
+    import bqcd_hip
+
+    for bad in (torch.zeros(48, dtype=torch.float64), np.zeros(48)):  # host memory
+        with pytest.raises(bqcd_hip.HipError, match="float64 GPU tensor"):
+            bqcd_hip._tp(bad)
+
+    class FakeDevice:  # stands in for a GPU tensor: only its description is read
+        is_cuda = True
+        shape = (12, 2, 2)
+
+        def __init__(self, dtype, contiguous=True):
+            self.dtype, self.contiguous = dtype, contiguous
+
+        def is_contiguous(self):
+            return self.contiguous
+
+        def numel(self):
+            return 48
+
+        def data_ptr(self):
+            return 4096
+
+    with pytest.raises(bqcd_hip.HipError, match="float64 GPU tensor"):
+        bqcd_hip._tp(FakeDevice(torch.float32))
+    with pytest.raises(bqcd_hip.HipError, match="float64 GPU tensor"):
+        bqcd_hip._tp(FakeDevice(torch.float64, contiguous=False))
+    with pytest.raises(bqcd_hip.HipError, match="48 values where 96"):
+        bqcd_hip._tp(FakeDevice(torch.float64), 96)
+    assert bqcd_hip._tp(FakeDevice(torch.float64), 48).value == 4096
