@@ -1,0 +1,164 @@
+"""Pins the oracle's hopping matrix d / d_dag (d/D21.F90:29-76) by what the
+mathematics fixes, independently of the reference's golden vectors.
+
+On unit links a plane wave in(x) = exp(i p.x) chi is an eigenfunction of the
+hop: out(x) = exp(i p.x) K(p) chi with
+
+    K(p) = sum_mu [(1 - g_mu) exp(i p_mu) + (1 + g_mu) exp(-i p_mu)]
+         = 2 sum_mu cos p_mu - 2 i sum_mu sin p_mu g_mu,
+
+and d_dag the same with g_mu -> -g_mu.  The g_mu read off the oracle must be
+Hermitian, square to one, anticommute pairwise, be traceless, and act on spin
+only (identity in colour); K must be linear in the sin p_mu.  With the links
+of the last time slice negated (flip_bc) the field is antiperiodic in time,
+and the momenta p_t = (2n + 1) pi / L_t are the eigenmodes.  A dropped hop,
+a projector of the wrong sign, a direction that moves along another axis, a
+transposed colour product or a missing boundary sign each break one of these.
+"""
+import numpy as np
+import pytest
+
+import oracle_lib as ol
+
+L = (4, 6, 4, 8)
+
+
+def coords(eo):
+    """coordinates (x, y, z, t) of the sites of parity eo in the oracle's order
+    i = (x + Lx (y + Ly (z + Lz t))) / 2"""
+    lx, ly, lz, lt = L
+    n = np.arange(lx * ly * lz * lt)
+    x, y, z, t = n % lx, (n // lx) % ly, (n // (lx * ly)) % lz, n // (lx * ly * lz)
+    m = (x + y + z + t) % 2 == eo
+    c = np.stack([x[m], y[m], z[m], t[m]], 1)
+    return c[np.argsort(n[m] // 2, kind="stable")]
+
+
+def unit_links(lat):
+    u = lat.gauge()
+    for k in range(3):
+        u[:, :, :, k, k, 0] = 1.0
+    return u
+
+
+def hop_matrix(lat, u, p, dagger, e=0):
+    """K(p) (12 x 12, index 4 * colour + spin) from the oracle's d on 12 plane
+    waves, checking that each output is exp(i p.x) times one constant spinor"""
+    pe = np.exp(1j * coords(e) @ p)
+    po = np.exp(1j * coords(1 - e) @ p)
+    K = np.zeros((12, 12), complex)
+    for k in range(12):
+        chi = np.zeros(12, complex)
+        chi[k] = 1.0
+        inp = (po[:, None] * chi[None, :]).reshape(-1, 3, 4)
+        out = lat.d(e, dagger, np.stack([inp.real, inp.imag], -1), u)
+        v = (out[..., 0] + 1j * out[..., 1]).reshape(-1, 12) / pe[:, None]
+        assert np.max(np.abs(v - v[0])) < 1e-12, "not an eigenfunction"
+        K[:, k] = v[0]
+    return K
+
+
+def momentum(n, antiperiodic=False):
+    p = 2 * np.pi * np.asarray(n, float) / np.asarray(L, float)
+    if antiperiodic:
+        p[3] += np.pi / L[3]
+    return p
+
+
+@pytest.fixture(scope="module")
+def gammas():
+    """g_mu read off the oracle: K(p) for p along mu alone"""
+    lat = ol.Lattice(L)
+    u = unit_links(lat)
+    g = []
+    for mu in range(4):
+        n = [0, 0, 0, 0]
+        n[mu] = 1
+        p = momentum(n)
+        K = hop_matrix(lat, u, p, 0)
+        scalar = 2 * np.cos(p).sum()
+        g.append((K - scalar * np.eye(12)) / (-2j * np.sin(p[mu])))
+    return g
+
+
+def test_zero_momentum_is_eight():
+    lat = ol.Lattice(L)
+    K = hop_matrix(lat, unit_links(lat), np.zeros(4), 0)
+    assert np.max(np.abs(K - 8 * np.eye(12))) < 1e-13
+
+
+def test_gammas_form_a_clifford_algebra(gammas):
+    one = np.eye(12)
+    for mu, gm in enumerate(gammas):
+        g4 = gm.reshape(3, 4, 3, 4)
+        # colour identity: only the (c, c) blocks are non-zero and they are equal
+        for c in range(3):
+            for c2 in range(3):
+                want = g4[0, :, 0, :] if c == c2 else np.zeros((4, 4))
+                assert np.max(np.abs(g4[c, :, c2, :] - want)) < 1e-12
+        assert np.max(np.abs(gm - gm.conj().T)) < 1e-12      # Hermitian
+        assert np.max(np.abs(gm @ gm - one)) < 1e-12         # squares to one
+        assert abs(np.trace(gm)) < 1e-12                     # traceless
+        for nu in range(mu):
+            ac = gm @ gammas[nu] + gammas[nu] @ gm
+            assert np.max(np.abs(ac)) < 1e-12, (mu, nu)      # anticommute
+
+
+@pytest.mark.parametrize("antiperiodic", [False, True])
+def test_general_momentum_and_dagger(gammas, antiperiodic):
+    lat = ol.Lattice(L)
+    u = unit_links(lat)
+    if antiperiodic:
+        u = lat.flip_bc(u)
+    p = momentum([1, 2, 3, 1], antiperiodic)
+    scalar = 2 * np.cos(p).sum() * np.eye(12)
+    slash = sum(np.sin(p[mu]) * gammas[mu] for mu in range(4))
+    for dagger, sign in ((0, -1), (1, 1)):
+        for e in (0, 1):
+            K = hop_matrix(lat, u, p, dagger, e)
+            assert np.max(np.abs(K - (scalar + sign * 2j * slash))) < 1e-12, (dagger, e)
+
+
+def constant_links(lat, V):
+    """U_mu(x) = V[mu] everywhere; host layout [mu][eo][site][col][row][re, im]
+    (BQCD's u(row, col, site, eo, mu))"""
+    u = lat.gauge()
+    for mu in range(4):
+        u[mu, :, :, :, :, 0] = V[mu].T.real
+        u[mu, :, :, :, :, 1] = V[mu].T.imag
+    return u
+
+
+@pytest.mark.parametrize("antiperiodic", [False, True])
+def test_constant_su3_links(gammas, antiperiodic):
+    """U_mu(x) = V_mu: K = sum_mu exp(i p_mu) V_mu (1 - g_mu) + exp(-i p_mu) V_mu^+ (1 + g_mu),
+    colour (slow index) times spin; catches a transposed or un-daggered link"""
+    from scipy.linalg import expm
+
+    rng = np.random.default_rng(7)
+    V = []
+    for mu in range(4):
+        h = rng.normal(size=(3, 3)) + 1j * rng.normal(size=(3, 3))
+        h = 0.5 * (h + h.conj().T)
+        h -= np.trace(h) / 3 * np.eye(3)
+        V.append(expm(1j * 0.7 * h))
+    lat = ol.Lattice(L)
+    u = constant_links(lat, V)
+    if antiperiodic:
+        u = lat.flip_bc(u)
+    p = momentum([1, 0, 2, 3], antiperiodic)
+    one = np.eye(12)
+    for dagger in (0, 1):
+        s = 1 if dagger else -1   # d: (1 - g) forward; d_dag: (1 + g) forward
+        want = sum(np.exp(1j * p[mu]) * np.kron(V[mu], np.eye(4)) @ (one + s * gammas[mu]) +
+                   np.exp(-1j * p[mu]) * np.kron(V[mu].conj().T, np.eye(4)) @ (one - s * gammas[mu])
+                   for mu in range(4))
+        K = hop_matrix(lat, u, p, dagger)
+        assert np.max(np.abs(K - want)) < 1e-12, dagger
+
+
+def test_antiperiodic_needs_the_flipped_links():
+    """the half-integer time momenta are eigenmodes only with the boundary sign"""
+    lat = ol.Lattice(L)
+    with pytest.raises(AssertionError, match="eigenfunction"):
+        hop_matrix(lat, unit_links(lat), momentum([0, 0, 0, 0], True), 0)
