@@ -162,3 +162,78 @@ def test_antiperiodic_needs_the_flipped_links():
     lat = ol.Lattice(L)
     with pytest.raises(AssertionError, match="eigenfunction"):
         hop_matrix(lat, unit_links(lat), momentum([0, 0, 0, 0], True), 0)
+
+
+def full_lattice_links(u):
+    """U[mu][x, y, z, t] as 3 x 3 (row, col) matrices from the host layout"""
+    U = np.zeros((4,) + L + (3, 3), complex)
+    for eo in range(2):
+        c = coords(eo)
+        for mu in range(4):
+            m = u[mu, eo, :, :, :, 0] + 1j * u[mu, eo, :, :, :, 1]   # [site][col][row]
+            U[mu, c[:, 0], c[:, 1], c[:, 2], c[:, 3]] = m.transpose(0, 2, 1)
+    return U
+
+
+def field_strength(U, mu, nu):
+    """F_mu nu = (Q - Q^+) / 8i with Q the sum of the four plaquette leaves at x
+    in the mu-nu plane, all oriented mu before nu"""
+    def at(X, *steps):  # X(x + sum of steps), a step being (direction, +-1)
+        for d, s in steps:
+            X = np.roll(X, -s, axis=d)
+        return X
+
+    def dag(X):
+        return np.conj(np.swapaxes(X, -1, -2))
+
+    Um, Un = U[mu], U[nu]
+    q = (Um @ at(Un, (mu, 1)) @ dag(at(Um, (nu, 1))) @ dag(Un)
+         + Un @ dag(at(Um, (mu, -1), (nu, 1))) @ dag(at(Un, (mu, -1))) @ at(Um, (mu, -1))
+         + dag(at(Um, (mu, -1))) @ dag(at(Un, (mu, -1), (nu, -1))) @ at(Um, (mu, -1), (nu, -1)) @ at(Un, (nu, -1))
+         + dag(at(Un, (nu, -1))) @ at(Um, (nu, -1)) @ at(Un, (mu, 1), (nu, -1)) @ dag(Um))
+    return (q - dag(q)) / 8j
+
+
+def site_matrices(lat, apply):
+    """the 12 x 12 matrix (index 4 * colour + spin) of a site-local operator on every site"""
+    n = lat.volh
+    M = np.zeros((n, 12, 12), complex)
+    for k in range(12):
+        chi = np.zeros((n, 12), complex)
+        chi[:, k] = 1.0
+        out = apply(np.stack([chi.real, chi.imag], -1).reshape(n, 3, 4, 2))
+        M[:, :, k] = (out[..., 0] + 1j * out[..., 1]).reshape(n, 12)
+    return M
+
+
+@pytest.mark.parametrize("e", [0, 1])
+def test_clover_term_closed_form(gammas, e):
+    """clover_init (clover/clover_init.F90:28-189): on every site
+    A = 1 - csw kappa sum_{mu<nu} sigma_mu nu F_mu nu, sigma_mu nu = (i/2)[g_mu, g_nu],
+    with the g_mu of the hopping matrix and F built here from the links; the
+    stored inverse is A^-1, the LDL^+ substitution (clover_mult_b) applies the
+    same A^-1, A is Hermitian with trace 12 and commutes with g_5"""
+    lat = ol.Lattice(L)
+    u = lat.gauge_exp_random(11, 0.4)
+    ck = 0.25
+    a, ai, b = lat.clover_init(u, ck)
+    A = site_matrices(lat, lambda x: lat.clover_mult_a(a[e], x))
+    Ai = site_matrices(lat, lambda x: lat.clover_mult_a(ai[e], x))
+    U = full_lattice_links(u)
+    c = coords(e)
+    S = np.zeros_like(A)
+    for mu in range(4):
+        for nu in range(mu + 1, 4):
+            F = field_strength(U, mu, nu)[c[:, 0], c[:, 1], c[:, 2], c[:, 3]]
+            sigma = 0.5j * (gammas[mu] @ gammas[nu] - gammas[nu] @ gammas[mu])[:4, :4]
+            S += np.einsum("nab,st->nasbt", F, sigma).reshape(lat.volh, 12, 12)
+    one = np.eye(12)
+    assert np.max(np.abs(A - one)) > 0.05                     # a non-trivial field
+    assert np.max(np.abs(A - (one - ck * S))) < 1e-13
+    assert np.max(np.abs(Ai @ A - one)) < 1e-13
+    assert np.max(np.abs(A - np.conj(np.swapaxes(A, 1, 2)))) < 1e-14
+    assert np.max(np.abs(np.trace(A, axis1=1, axis2=2) - 12)) < 1e-13
+    g5 = gammas[0] @ gammas[1] @ gammas[2] @ gammas[3]
+    assert np.max(np.abs(A @ g5 - g5 @ A)) < 1e-14
+    chi = np.random.default_rng(3).normal(size=(lat.volh, 3, 4, 2))
+    assert np.max(np.abs(lat.clover_mult_b(b[e], chi) - lat.clover_mult_a(ai[e], chi))) < 1e-13

@@ -1,4 +1,4 @@
-"""The closed-form checks of the oracle's hopping matrix
+"""The closed-form checks of the oracle's hopping matrix and clover term
 (test_oracle_closed_form.py) catch plausible mistakes: each case below plants
 one mistake in a copy of oracle/bqcd_oracle.c, builds it in a temporary
 directory, and runs those checks against it, which must fail.  CPU only; the
@@ -28,6 +28,16 @@ MUTANTS = {
     "t projector factor 2 dropped": ("b[up][c] += 2.0 * w[c][0];", "b[up][c] += w[c][0];"),
     "t projector spins swapped": ("int up = (s < 0) ? 2 : 0; /* (1 - g_t)", "int up = (s < 0) ? 0 : 2; /* (1 - g_t)"),
     "lower half sign": ("b[2 + k][c] += s * (conj(", "b[2 + k][c] += -s * (conj("),
+    # clover_init, clover_f_mu_nu, clover_inv
+    "clover second block sign": ("cplx v2 = factor * (P[r][c] - Q[r][c]);",
+                                 "cplx v2 = factor * (P[r][c] + Q[r][c]);"),
+    "clover off-diagonal phase": ("P[r][c + 3] = f[r][c] + I * h[r][c];",
+                                  "P[r][c + 3] = f[r][c] - I * h[r][c];"),
+    "clover leaf link conjugated": ("load_u(c, u, volh, nu, e, xpm, 0); load_u(d, u, volh, mu, e, x, 1);",
+                                    "load_u(c, u, volh, nu, e, xpm, 1); load_u(d, u, volh, mu, e, x, 1);"),
+    "clover plane 2-4 for 3-4": ("clover_f_mu_nu(g, f, 2, 3, i, eo, u);", "clover_f_mu_nu(g, f, 1, 3, i, eo, u);"),
+    "clover inverse row not conjugated": ("aset(ainv, col, c, 0.5 * conj(x[c]));", "aset(ainv, col, c, 0.5 * x[c]);"),
+    "clover LDL conjugate dropped": ("acc -= L[r][k] * d[k] * conj(L[c][k]);", "acc -= L[r][k] * d[k] * L[c][k];"),
 }
 
 RUN = """
