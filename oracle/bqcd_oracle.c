@@ -574,11 +574,12 @@ static void u_normalize(cplx m[3][3]) {
   m[2][2] = conj(m[0][0] * m[1][1] - m[1][0] * m[0][1]);
 }
 
-/* u := exp(i sum_j h_j lambda_j).  The reference's closed form (su3.F90:28-193)
- * is restated here by eigen-decomposition of the Hermitian traceless matrix
- * H = sum h_j lambda_j via the trigonometric cubic solution (Cardano), then
- * exp(iH) = sum_k e^{i l_k} P_k with P_k = prod_{m!=k} (H - l_m)/(l_k - l_m),
- * with a Taylor fallback for near-degenerate eigenvalues. */
+/* u := exp(i sum_j h_j lambda_j), lambda_j the Gell-Mann matrices.  The
+ * reference evaluates the exponential in closed form (su3.F90:28-193); here it
+ * is the Taylor series of exp(i H / 2^s) to 24 terms, with H scaled until the
+ * sum of its element magnitudes is at most 1/4, squared s times.  Both are the
+ * exact exponential to rounding; the golden hmc_u vectors and a comparison
+ * with a library matrix exponential check it. */
 static void gellmann_h(cplx H[3][3], const double h[8]) {
   const double s3 = 1.0 / 1.7320508075688772;
   H[0][0] = h[2] + s3 * h[7];

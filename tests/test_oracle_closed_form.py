@@ -237,3 +237,39 @@ def test_clover_term_closed_form(gammas, e):
     assert np.max(np.abs(A @ g5 - g5 @ A)) < 1e-14
     chi = np.random.default_rng(3).normal(size=(lat.volh, 3, 4, 2))
     assert np.max(np.abs(lat.clover_mult_b(b[e], chi) - lat.clover_mult_a(ai[e], chi))) < 1e-13
+
+
+def gell_mann():
+    lam = np.zeros((8, 3, 3), complex)
+    lam[0][0, 1] = lam[0][1, 0] = 1
+    lam[1][0, 1], lam[1][1, 0] = -1j, 1j
+    lam[2][0, 0], lam[2][1, 1] = 1, -1
+    lam[3][0, 2] = lam[3][2, 0] = 1
+    lam[4][0, 2], lam[4][2, 0] = -1j, 1j
+    lam[5][1, 2] = lam[5][2, 1] = 1
+    lam[6][1, 2], lam[6][2, 1] = -1j, 1j
+    lam[7] = np.diag([1.0, 1.0, -2.0]) / np.sqrt(3.0)
+    return lam
+
+
+def test_hmc_u_is_the_exponential_map():
+    """hmc_u (hmc_u.F90:28-69, gen2u su3.F90:28-193): U' = exp(i step sum_j p_j
+    lambda_j) U for every link, with the standard Gell-Mann matrices (the
+    generator coordinates of im_tr_j / re_tr_j, su3.F90:196-233), compared with
+    scipy's matrix exponential; the updated links are SU(3)"""
+    from scipy.linalg import expm
+
+    lat = ol.Lattice((4, 4, 4, 4))
+    u = lat.gauge_exp_random(5, 0.5)
+    p = np.random.default_rng(1).normal(size=lat.gen().shape)
+    step = 0.3
+    un = lat.hmc_u(u, p, step)
+    lam = gell_mann()
+    m0 = (u[..., 0] + 1j * u[..., 1]).swapaxes(-1, -2)     # (mu, eo, site, row, col)
+    m1 = (un[..., 0] + 1j * un[..., 1]).swapaxes(-1, -2)
+    H = np.einsum("meij,jab->meiab", p, lam)
+    want = np.array([[[expm(1j * step * H[m, e, i]) @ m0[m, e, i] for i in range(lat.volh)]
+                      for e in range(2)] for m in range(4)])
+    assert np.max(np.abs(m1 - want)) < 1e-13
+    assert np.max(np.abs(m1.conj().swapaxes(-1, -2) @ m1 - np.eye(3))) < 1e-14
+    assert np.max(np.abs(np.linalg.det(m1) - 1.0)) < 1e-14

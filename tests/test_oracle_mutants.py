@@ -1,5 +1,5 @@
-"""The closed-form checks of the oracle's hopping matrix and clover term
-(test_oracle_closed_form.py) catch plausible mistakes: each case below plants
+"""The closed-form checks of the oracle's hopping matrix, clover term and
+link update (test_oracle_closed_form.py) catch plausible mistakes: each case below plants
 one mistake in a copy of oracle/bqcd_oracle.c, builds it in a temporary
 directory, and runs those checks against it, which must fail.  CPU only; the
 oracle itself is untouched."""
@@ -38,6 +38,14 @@ MUTANTS = {
     "clover plane 2-4 for 3-4": ("clover_f_mu_nu(g, f, 2, 3, i, eo, u);", "clover_f_mu_nu(g, f, 1, 3, i, eo, u);"),
     "clover inverse row not conjugated": ("aset(ainv, col, c, 0.5 * conj(x[c]));", "aset(ainv, col, c, 0.5 * x[c]);"),
     "clover LDL conjugate dropped": ("acc -= L[r][k] * d[k] * conj(L[c][k]);", "acc -= L[r][k] * d[k] * L[c][k];"),
+    # hmc_u: generator matrix, exponential, reunitarisation
+    "lambda_2 sign": ("H[0][1] = h[0] - I * h[1];", "H[0][1] = h[0] + I * h[1];"),
+    "lambda_8 normalisation": ("H[2][2] = -2.0 * s3 * h[7];", "H[2][2] = -s3 * h[7];"),
+    "exponential series truncated": ("for (int k = 1; k <= 24; k++) {", "for (int k = 1; k <= 3; k++) {"),
+    "half step": ("for (int j = 0; j < 8; j++) q[j] = pp[j] * step;",
+                  "for (int j = 0; j < 8; j++) q[j] = pp[j] * 0.5 * step;"),
+    "third column not conjugated": ("m[0][2] = conj(m[1][0] * m[2][1] - m[2][0] * m[1][1]);",
+                                    "m[0][2] = (m[1][0] * m[2][1] - m[2][0] * m[1][1]);"),
 }
 
 RUN = """
