@@ -1,5 +1,5 @@
 """The oracle's checks that stand without the reference -- the closed forms of
-the hopping matrix, clover term and link update (test_oracle_closed_form.py)
+the hopping matrix, clover term, link update and CG (test_oracle_closed_form.py)
 and the finite-difference gradients of the forces (test_oracle_fd.py) --
 catch plausible mistakes: each case below plants one mistake in a copy of the
 oracle's sources, builds it in a temporary directory, and runs those checks
@@ -54,6 +54,10 @@ MUTANTS = {
                   "for (int j = 0; j < 8; j++) q[j] = pp[j] * 0.5 * step;"),
     "third column not conjugated": (C, "m[0][2] = conj(m[1][0] * m[2][1] - m[2][0] * m[1][1]);",
                                     "m[0][2] = (m[1][0] * m[2][1] - m[2][0] * m[1][1]);"),
+    # operators and cg
+    "operator kappa not squared": (C, "double bfac = -kappa * kappa;", "double bfac = -kappa;"),
+    "cg beta inverted": (C, "double bk = rtr / rtrold;", "double bk = rtrold / rtr;"),
+    "cg x updated along r": (C, "x[k] += ak * p[k];", "x[k] += ak * r[k];"),
     # forces: im_tr_j / re_tr_j, dsg staples, dsf hopping and clover parts
     "im_tr_j component sign": (C, "p[1] += s * (creal(v[0][1]) - creal(v[1][0]));",
                                "p[1] += s * (creal(v[1][0]) - creal(v[0][1]));"),
