@@ -1,7 +1,9 @@
 """C-ABI checks that need no GPU: the library loads (its HIP runtime
 dependency resolves without a device) and exports every entry point that
-include/bqcd_hip.h declares, and nothing on the host side falls back to the
-oracle when the library is absent."""
+include/bqcd_hip.h declares; the Fortran module's interfaces and the HMC
+structs of the Fortran module and the ctypes harness match the header;
+device tensors are checked before they reach the library; and nothing on
+the host side falls back to the oracle when the library is absent."""
 import ctypes
 import os
 import re
@@ -94,3 +96,68 @@ def test_device_tensors_are_checked_before_they_reach_the_library():
     with pytest.raises(bqcd_hip.HipError, match="48 values where 96"):
         bqcd_hip._tp(FakeDevice(torch.float64), 96)
     assert bqcd_hip._tp(FakeDevice(torch.float64), 48).value == 4096
+
+
+FORTRAN = os.path.join(REPO, "bqcd-r399-quda_amd", "fortran", "bqcd_hip.F90")
+
+
+def header_text():
+    return re.sub(r"/\*.*?\*/", "", open(HEADER).read(), flags=re.S)
+
+
+def fortran_text():
+    return re.sub(r"&[ \t]*\n\s*", " ", open(FORTRAN).read())   # continuation lines joined
+
+
+def test_fortran_interfaces_match_the_header():
+    """every bind(C) function of the Fortran module names a declared entry point
+    and passes as many arguments as the header declares"""
+    protos = {}
+    for m in re.finditer(r"^\s*(?:const\s+)?[\w ]+?\**\s*(hb_\w+)\s*\(([^;]*?)\)\s*;", header_text(), flags=re.M):
+        args = m.group(2).strip()
+        protos[m.group(1)] = 0 if args in ("", "void") else len(args.split(","))
+    binds = {}
+    pat = r"(?:function|subroutine)[ \t]+\w+[ \t]*\(([^)]*)\)[^\n]*?bind\(C,\s*name=\"(hb_\w+)\"\)"
+    for m in re.finditer(pat, fortran_text()):
+        binds[m.group(2)] = len(m.group(1).split(",")) if m.group(1).strip() else 0
+    assert len(binds) >= 30
+    for name, n in binds.items():
+        assert name in protos, name
+        assert n == protos[name], (name, n, protos[name])
+
+
+def c_struct(name):
+    body = re.search(r"typedef struct \{([^{}]*)\}\s*" + name + r"\s*;", header_text()).group(1)
+    kinds = {"double": "d", "int": "i", "char": "c", "float": "f"}
+    out = []
+    for decl in body.split(";"):
+        decl = decl.strip()
+        if decl:
+            t, rest = decl.split(None, 1)
+            out += [(n.strip(), kinds[t]) for n in rest.split(",")]
+    return out
+
+
+def fortran_type(name):
+    body = re.search(r"type, bind\(C\) :: " + name + r"\n(.*?)end type", fortran_text(), flags=re.S).group(1)
+    kinds = {"real(c_double)": "d", "integer(c_int)": "i", "character(kind=c_char)": "c", "real(c_float)": "f"}
+    out = []
+    for line in body.strip().splitlines():
+        t, names = line.split("::")
+        out += [(n.strip(), kinds[t.strip()]) for n in names.split(",")]
+    return out
+
+
+@pytest.mark.parametrize("name", ["hb_hmc_params", "hb_hmc_out"])
+def test_struct_mirrors_match_the_header(name):
+    """the HMC parameter and result structs have the header's members, in order
+    and with the same C types, in the Fortran module and in the ctypes harness"""
+    import ctypes as ct
+    import bqcd_hip
+
+    c = c_struct(name)
+    assert fortran_type(name) == c
+    py = {"hb_hmc_params": bqcd_hip.HmcParams, "hb_hmc_out": bqcd_hip.HmcOut}[name]
+    kinds = {ct.c_double: "d", ct.c_int: "i", ct.c_char: "c", ct.c_float: "f"}
+    rename = {"lam": "lambda"}   # a Python keyword
+    assert [(rename.get(n, n), kinds[t]) for n, t in py._fields_] == c
