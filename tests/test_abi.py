@@ -1,7 +1,7 @@
 """C-ABI checks that need no GPU: the library loads (its HIP runtime
 dependency resolves without a device) and exports every entry point that
-include/bqcd_hip.h declares; the Fortran module's interfaces and the HMC
-structs of the Fortran module and the ctypes harness match the header;
+include/bqcd_hip.h declares; the Fortran module's interfaces, the ctypes
+argument lists and the HMC structs of both bindings match the header;
 device tensors are checked before they reach the library; and nothing on
 the host side falls back to the oracle when the library is absent."""
 import ctypes
@@ -161,3 +161,42 @@ def test_struct_mirrors_match_the_header(name):
     kinds = {ct.c_double: "d", ct.c_int: "i", ct.c_char: "c", ct.c_float: "f"}
     rename = {"lam": "lambda"}   # a Python keyword
     assert [(rename.get(n, n), kinds[t]) for n, t in py._fields_] == c
+
+
+def split_top_level(s):
+    """s split at the commas outside any bracket"""
+    parts, depth, cur = [], 0, ""
+    for ch in s:
+        if ch in "([":
+            depth += 1
+        elif ch in ")]":
+            depth -= 1
+        if ch == "," and depth == 0:
+            parts.append(cur)
+            cur = ""
+        else:
+            cur += ch
+    return [p for p in parts + [cur] if p.strip()]
+
+
+def test_ctypes_argtypes_match_the_header():
+    """every argtypes list of the ctypes harness has the header's argument count"""
+    protos = {}
+    for m in re.finditer(r"^\s*(?:const\s+)?[\w ]+?\**\s*(hb_\w+)\s*\(([^;]*?)\)\s*;", header_text(), flags=re.M):
+        args = m.group(2).strip()
+        protos[m.group(1)] = 0 if args in ("", "void") else len(args.split(","))
+    src = open(os.path.join(REPO, "bqcd-r399-quda_amd", "bqcd_hip.py")).read()
+    seen = 0
+    for m in re.finditer(r"L\.(hb_\w+)\.argtypes\s*=\s*\[", src):
+        i = j = m.end()
+        depth = 1
+        while depth:
+            if src[j] in "([":
+                depth += 1
+            elif src[j] in ")]":
+                depth -= 1
+            j += 1
+        n = len(split_top_level(src[i:j - 1]))
+        assert protos.get(m.group(1)) == n, (m.group(1), n, protos.get(m.group(1)))
+        seen += 1
+    assert seen >= len(protos) - 1   # all but the argument-less hb_last_error
