@@ -275,24 +275,26 @@ def test_hmc_u_is_the_exponential_map():
     assert np.max(np.abs(np.linalg.det(m1) - 1.0)) < 1e-14
 
 
-def test_cg_on_a_plane_wave_converges_in_two_iterations():
-    """cg (cg.F90:51-137) on M~^+ M~ (mtdagmt.F90, m_tilde.F90), unit links,
-    csw = 0: M~ = 1 - kappa^2 D_eo D_oe maps exp(i p.x) chi to exp(i p.x)
-    (1 - kappa^2 K^2) chi, and M~^+ uses K^+ (d_dag).  K = a - i b.g has two
-    eigenvalues, so M~^+ M~ has two on this invariant space and CG ends after
-    two iterations with x = exp(i p.x) (M^+ M)^-1 chi."""
+@pytest.mark.parametrize("which,rho", [("mtdagmt", 0.0), ("w_dagger_w", 0.1)])
+def test_cg_on_a_plane_wave_converges_in_two_iterations(which, rho):
+    """cg (cg.F90:51-137) on M~^+ M~ (mtdagmt.F90, m_tilde.F90) and on W^+ W
+    with W = M~ + rho (w_mult.F90), unit links, csw = 0: M~ = 1 - kappa^2 D_eo
+    D_oe maps exp(i p.x) chi to exp(i p.x) (1 - kappa^2 K^2) chi, and M~^+
+    uses K^+ (d_dag).  K = a - i b.g has two eigenvalues, so the normal
+    operator has two on this invariant space and CG ends after two iterations
+    with x = exp(i p.x) (M^+ M)^-1 chi, M = 1 - kappa^2 K^2 + rho."""
     lat = ol.Lattice(L)
     u = unit_links(lat)
     a, ai, b = lat.clover_init(u, 0.0)
     kappa = 0.12
-    op = lat.op(u, a, ai, kappa, 0.0)
+    op = lat.op(u, a, ai, kappa, 0.0, rho)
     p = momentum([1, 2, 1, 3])
     K = hop_matrix(lat, u, p, 0)
-    M = np.eye(12) - kappa ** 2 * K @ K
+    M = (1.0 + rho) * np.eye(12) - kappa ** 2 * K @ K
     chi = np.random.default_rng(5).normal(size=12) + 1j * np.random.default_rng(6).normal(size=12)
     pe = np.exp(1j * coords(0) @ p)
     src = (pe[:, None] * chi[None, :]).reshape(-1, 3, 4)
-    x, iters, rc = lat.cg(op, ol.OPS["mtdagmt"], np.stack([src.real, src.imag], -1), 1e-24, 50)
+    x, iters, rc = lat.cg(op, ol.OPS[which], np.stack([src.real, src.imag], -1), 1e-24, 50)
     assert rc == 0 and iters <= 2, iters
     want = pe[:, None] * np.linalg.solve(M.conj().T @ M, chi)[None, :]
     got = (x[..., 0] + 1j * x[..., 1]).reshape(-1, 12)

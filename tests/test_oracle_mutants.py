@@ -58,6 +58,8 @@ MUTANTS = {
     "operator kappa not squared": (C, "double bfac = -kappa * kappa;", "double bfac = -kappa;"),
     "cg beta inverted": (C, "double bk = rtr / rtrold;", "double bk = rtrold / rtr;"),
     "cg x updated along r": (C, "x[k] += ak * p[k];", "x[k] += ak * r[k];"),
+    "W shift with the wrong sign": (C, "for (int k = 0; k < n; k++) out[k] += op->p.rho * in[k];",
+                                    "for (int k = 0; k < n; k++) out[k] -= op->p.rho * in[k];"),
     # forces: im_tr_j / re_tr_j, dsg staples, dsf hopping and clover parts
     "im_tr_j component sign": (C, "p[1] += s * (creal(v[0][1]) - creal(v[1][0]));",
                                "p[1] += s * (creal(v[1][0]) - creal(v[0][1]));"),
