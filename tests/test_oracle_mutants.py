@@ -116,16 +116,29 @@ def build_and_check(d, mutation=None):
 needs_gcc = pytest.mark.skipif(shutil.which("gcc") is None, reason="gcc not available")
 
 
+@pytest.fixture(scope="module")
+def results(tmp):
+    """every build and check run, a few at a time (each is its own process)"""
+    from concurrent.futures import ThreadPoolExecutor
+
+    jobs = {"control": (os.path.join(tmp, "control"), None)}
+    for k, name in enumerate(sorted(MUTANTS)):
+        jobs[name] = (os.path.join(tmp, "m%d" % k), MUTANTS[name])
+    with ThreadPoolExecutor(max_workers=min(8, os.cpu_count() or 1)) as pool:
+        futures = {name: pool.submit(build_and_check, *job) for name, job in jobs.items()}
+        return {name: f.result() for name, f in futures.items()}
+
+
 @needs_gcc
-def test_unmodified_copy_passes(tmp):
+def test_unmodified_copy_passes(results):
     """the control: the same copy-and-build without a mistake passes every check"""
-    r = build_and_check(os.path.join(tmp, "control"))
+    r = results["control"]
     assert r.returncode == 0, r.stdout[-2000:]
 
 
 @needs_gcc
 @pytest.mark.parametrize("name", sorted(MUTANTS))
-def test_checks_fail_on_planted_mistake(tmp, name):
-    r = build_and_check(os.path.join(tmp, "m%d" % sorted(MUTANTS).index(name)), MUTANTS[name])
+def test_checks_fail_on_planted_mistake(results, name):
+    r = results[name]
     assert r.returncode == 1, r.stdout[-2000:]   # 1: the checks ran and some failed
     assert " failed" in r.stdout
