@@ -42,7 +42,7 @@ class HmcParams(ctypes.Structure):
                 ("rho", ctypes.c_double), ("tau", ctypes.c_double), ("ntau", ctypes.c_int),
                 ("m_scale", ctypes.c_int), ("model", ctypes.c_char), ("integrator", ctypes.c_int),
                 ("lam", ctypes.c_double), ("rest", ctypes.c_float), ("maxiter", ctypes.c_int),
-                ("precision", ctypes.c_int), ("delta", ctypes.c_double), ("max_inner", ctypes.c_int),
+                ("log", ctypes.c_int), ("precision", ctypes.c_int), ("delta", ctypes.c_double), ("max_inner", ctypes.c_int),
                 ("mre_vectors", ctypes.c_int), ("batch", ctypes.c_int), ("verbose", ctypes.c_int)]
 
 
@@ -61,11 +61,16 @@ class HmcOut(ctypes.Structure):
 INTEGRATOR = {"leapfrog": 0, "omelyan": 1}
 
 
+def multi_block_limit(volh, precision="f16"):
+    """largest multi-RHS block size whose block fields fit one buffer (host only)"""
+    return lib().hb_multi_block_limit(volh, PREC[precision])
+
+
 def hmc_params(beta, kappa, csw, rho, tau, ntau, m_scale=1, model="C", integrator="leapfrog",
                lam=0.0, rest=1e-20, maxiter=10000, precision="f64", delta=0.1, max_inner=1000,
-               mre_vectors=0, batch=False, verbose=False):
+               mre_vectors=0, batch=False, verbose=False, log=0):
     return HmcParams(beta, kappa, csw, rho, tau, ntau, m_scale, model.encode(), INTEGRATOR[integrator],
-                     lam, rest, maxiter, PREC[precision], delta, max_inner, mre_vectors, int(batch),
+                     lam, rest, maxiter, log, PREC[precision], delta, max_inner, mre_vectors, int(batch),
                      int(verbose))
 
 
@@ -117,6 +122,8 @@ def lib():
                                           ctypes.POINTER(HmcOut)]
         L.hb_hmc_test_d.argtypes = [_vp, ctypes.POINTER(HmcParams), ctypes.c_int, _vp, _vp, _vp,
                                     ctypes.POINTER(HmcOut)]
+        L.hb_hmc_test.argtypes = [_vp, ctypes.POINTER(HmcParams), ctypes.c_int, _dp, ctypes.c_int, _dp, _dp,
+                                  ctypes.POINTER(HmcOut)]
         L.hb_gaussian_d.argtypes = [_vp, ctypes.c_ulonglong, ctypes.c_double, _vp, ctypes.c_longlong]
         L.hb_conf_write.argtypes = [ctypes.c_int * 4, ctypes.c_int * 4, _dp, ctypes.c_int, ctypes.c_char_p,
                                     ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double,
@@ -138,6 +145,7 @@ def lib():
                                     ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, _ip, _dp]
         L.hb_multi_block_size.argtypes = [_vp]
         L.hb_multi_tile.argtypes = [_vp]
+        L.hb_multi_block_limit.argtypes = [ctypes.c_int, ctypes.c_int]
         L.hb_time_dslash_lp.argtypes = [_vp, ctypes.c_int, ctypes.c_int, _vp, _dp]
         L.hb_multi_set_tile.argtypes = [_vp, ctypes.c_int]
         L.hb_dsf.argtypes = [_vp, _dp, ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_double, _dp, _dp]
@@ -526,6 +534,18 @@ class Lattice:
                                    None if eta1 is None else self._s(eta1),
                                    None if eta2 is None else self._s(eta2),
                                    ctypes.byref(out)))
+        return out.as_dict()
+
+    def hmc_test(self, prm, direction, p, eta1=None, eta2=None):
+        """hmc_test_d on host arrays: p (BQCD momenta layout, float64, contiguous)
+        is overwritten with the final momenta"""
+        if not (isinstance(p, np.ndarray) and p.dtype == np.float64 and p.flags.c_contiguous):
+            raise HipError("hmc_test: p must be a C-contiguous float64 array (it is written back)")
+        out = HmcOut()
+        _check(lib().hb_hmc_test(self._h, ctypes.byref(prm), direction, _p(p), p.shape[2],
+                                 None if eta1 is None else _p(np.ascontiguousarray(eta1, dtype=np.float64)),
+                                 None if eta2 is None else _p(np.ascontiguousarray(eta2, dtype=np.float64)),
+                                 ctypes.byref(out)))
         return out.as_dict()
 
     def gaussian_d(self, t, seed, var):

@@ -79,7 +79,11 @@ int hb_init(const int L[4], const int bc[4], int device, hb_context **out) {
              bc[m]);
   }
   long long vol = (long long)L[0] * L[1] * L[2] * L[3];
-  HB_CHECK(vol / 2 < (1LL << 30), HB_E_ARG, "hb_init: lattice too large");
+  // every field is addressed through buffer resources with 32-bit byte
+  // offsets; the largest single-buffer field is an FP64 spinor (192 B per site)
+  HB_CHECK(192LL * (vol / 2) < (1LL << 32), HB_E_ARG,
+           "hb_init: lattice too large (%lld sites per parity; an FP64 spinor field must stay below 4 GiB)",
+           (long long)(vol / 2));
   int ndev = 0;
   HB_HIP(hipGetDeviceCount(&ndev));
   HB_CHECK(device >= 0 && device < ndev, HB_E_ARG, "hb_init: device %d of %d", device, ndev);
@@ -150,6 +154,14 @@ int hb_set_stream(hb_context *ctx, void *stream) {
 int hb_multi_block_size(hb_context *ctx) {
   Context *c = reinterpret_cast<Context *>(ctx);
   return c ? c->last_block : 0;
+}
+
+int hb_multi_block_limit(int volh, int precision) {
+  if (volh < 1 || (precision != HB_PREC_F32 && precision != HB_PREC_F16)) return 0;
+  static const int cand[5] = {12, 8, 4, 2, 1};
+  for (int nr : cand)
+    if (hb::multi_block_fits(volh, nr, precision)) return nr;
+  return 0;
 }
 
 int hb_multi_tile(hb_context *ctx) {

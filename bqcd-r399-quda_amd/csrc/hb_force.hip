@@ -649,10 +649,10 @@ int hb_clover_action(hb_context *ctx, double *s_det) {
 int hb_gauge_backup(hb_context *ctx, int slot) {
   Context *c = reinterpret_cast<Context *>(ctx);
   HB_TRY(check_ready(c));
-  HB_CHECK(slot >= 0 && slot < 4, HB_E_ARG, "hb_gauge_backup: slot %d (0..3)", slot);
+  HB_CHECK(slot >= 0 && slot < kGaugeBackups, HB_E_ARG, "hb_gauge_backup: slot %d (0..%d)", slot, kGaugeBackups - 1);
   HB_HIP(hipSetDevice(c->device));
   int rc = 0;
-  void *dst = scratch(c, SLOT_MULTI + 60 + slot, sizeof(double2) * 72 * (size_t)c->g.vh, &rc); HB_TRY(rc);
+  void *dst = scratch(c, SLOT_BACKUP + slot, sizeof(double2) * 72 * (size_t)c->g.vh, &rc); HB_TRY(rc);
   HB_HIP(hipMemcpyAsync(dst, c->gauge, sizeof(double2) * 72 * (size_t)c->g.vh, hipMemcpyDeviceToDevice, c->stream));
   return HB_OK;
 }
@@ -660,11 +660,11 @@ int hb_gauge_backup(hb_context *ctx, int slot) {
 int hb_gauge_restore(hb_context *ctx, int slot) {
   Context *c = reinterpret_cast<Context *>(ctx);
   HB_TRY(check_ready(c));
-  HB_CHECK(slot >= 0 && slot < 4, HB_E_ARG, "hb_gauge_restore: slot %d (0..3)", slot);
-  HB_CHECK((int)c->scratch.size() > SLOT_MULTI + 60 + slot && c->scratch[SLOT_MULTI + 60 + slot].p, HB_E_STATE,
+  HB_CHECK(slot >= 0 && slot < kGaugeBackups, HB_E_ARG, "hb_gauge_restore: slot %d (0..%d)", slot, kGaugeBackups - 1);
+  HB_CHECK((int)c->scratch.size() > SLOT_BACKUP + slot && c->scratch[SLOT_BACKUP + slot].p, HB_E_STATE,
            "hb_gauge_restore: slot %d never backed up", slot);
   HB_HIP(hipSetDevice(c->device));
-  HB_HIP(hipMemcpyAsync(c->gauge, c->scratch[SLOT_MULTI + 60 + slot].p, sizeof(double2) * 72 * (size_t)c->g.vh,
+  HB_HIP(hipMemcpyAsync(c->gauge, c->scratch[SLOT_BACKUP + slot].p, sizeof(double2) * 72 * (size_t)c->g.vh,
                         hipMemcpyDeviceToDevice, c->stream));
   c->gauge_version++;
   return HB_OK;

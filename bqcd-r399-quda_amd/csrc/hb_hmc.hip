@@ -32,10 +32,9 @@ int cg_multi(Context *c, int op, int nrhs, const double *kappa, const double *rh
 
 namespace {
 
-constexpr int kMaxMre = 16;
 constexpr int kRedGrid = 512;  // blocks of the vector reductions (fits the context's partials)
 enum {
-  SLOT_HMC_P = SLOT_MULTI + 100,
+  SLOT_HMC_P = SLOT_HMC,
   SLOT_HMC_PHI1,
   SLOT_HMC_PHI2,
   SLOT_HMC_A,
@@ -44,9 +43,10 @@ enum {
   SLOT_HMC_B2,
   SLOT_HMC_S,
   SLOT_HMC_RES,
-  SLOT_HMC_MV = SLOT_MULTI + 120,                // kMaxMre products M v_i
-  SLOT_HMC_MRE = SLOT_HMC_MV + kMaxMre           // 2 bases of kMaxMre vectors
+  SLOT_HMC_U0,     // links at the start of the trajectory (restored on rejection)
+  SLOT_HMC_LAST
 };
+static_assert(SLOT_HMC_LAST <= SLOT_HMC_MV, "HMC fields overlap the MRE products");
 
 // ---------------------------------------------------------------------------
 // vector kernels on n complex (double2) numbers
@@ -192,10 +192,10 @@ struct Traj {
     int it = 0;
     double rtr = 0.0;
     if (prm.precision == HB_PREC_F64) {
-      HB_TRY(cg_f64(c, op, prm.kappa, rho, x, src, prm.rest, prm.maxiter, 0, &it, &rtr));
+      HB_TRY(cg_f64(c, op, prm.kappa, rho, x, src, prm.rest, prm.maxiter, prm.log, &it, &rtr));
     } else {
       int nrel = 0;
-      HB_TRY(cg_mixed(c, op, prm.kappa, rho, x, src, prm.rest, prm.maxiter, 0, prm.precision,
+      HB_TRY(cg_mixed(c, op, prm.kappa, rho, x, src, prm.rest, prm.maxiter, prm.log, prm.precision,
                       prm.delta, prm.max_inner, &it, &nrel, &rtr));
     }
     count(it);
@@ -359,7 +359,7 @@ struct Traj {
     float rest[2] = {prm.rest, prm.rest};
     int it[2] = {0, 0};
     double tr[2];
-    HB_TRY(cg_multi(c, HB_OP_WDAGW, 2, kap, rh, clover ? -1.0 : 0.0, xs, bs, rest, prm.maxiter, 0,
+    HB_TRY(cg_multi(c, HB_OP_WDAGW, 2, kap, rh, clover ? -1.0 : 0.0, xs, bs, rest, prm.maxiter, prm.log,
                     prm.precision, 2, it, tr));
     count(it[0]);
     count(it[1]);
@@ -545,7 +545,12 @@ int trajectory(Context *c, const hb_hmc_params *prm, const double *p_d, const do
   }
 
   // start: backups and the clover term of the current links (hmc.F90:104-113)
-  if (accept_random >= 0.0) HB_TRY(hb_gauge_backup(reinterpret_cast<hb_context *>(c), 0));
+  // (the library's own slot: a caller's hb_gauge_backup slots are not touched)
+  double2 *u0 = nullptr;
+  if (accept_random >= 0.0) {
+    u0 = static_cast<double2 *>(scratch(c, SLOT_HMC_U0, sizeof(double2) * 72 * (size_t)vh, &rc)); HB_TRY(rc);
+    HB_HIP(hipMemcpyAsync(u0, c->gauge, sizeof(double2) * 72 * (size_t)vh, hipMemcpyDeviceToDevice, c->stream));
+  }
   if (T.clover) HB_TRY(build_clover_dev(c, T.ck));
   HB_HIP(hipMemcpyAsync(T.P, p_d, sizeof(double) * 64 * (size_t)vh, hipMemcpyDeviceToDevice, c->stream));
 
@@ -598,7 +603,8 @@ int trajectory(Context *c, const hb_hmc_params *prm, const double *p_d, const do
   o.sf2_new = T.sf2;
   o.accepted = (accept_random < 0.0 || accept_random < o.exp_dh) ? 1 : 0;
   if (!o.accepted) {
-    HB_TRY(hb_gauge_restore(reinterpret_cast<hb_context *>(c), 0));
+    HB_HIP(hipMemcpyAsync(c->gauge, u0, sizeof(double2) * 72 * (size_t)vh, hipMemcpyDeviceToDevice, c->stream));
+    c->gauge_version++;
     if (T.clover) HB_TRY(build_clover_dev(c, T.ck));
     HB_TRY(T.sg(&o.plaquette));
   } else {
@@ -660,6 +666,32 @@ int hb_hmc_trajectory(hb_context *ctx, const hb_hmc_params *prm, const double *p
   if (eta2) HB_TRY(spinor_upload(c, e2, eta2));
   return trajectory(c, prm, pd, reinterpret_cast<double *>(e1), eta2 ? reinterpret_cast<double *>(e2) : nullptr,
                     accept_random, out);
+}
+
+int hb_hmc_test(hb_context *ctx, const hb_hmc_params *prm, int direction, double *p, int site_stride,
+                const double *eta1, const double *eta2, hb_hmc_out *out) {
+  Context *c = reinterpret_cast<Context *>(ctx);
+  HB_TRY(check_ready(c));
+  HB_CHECK(direction == 1 || direction == -1, HB_E_ARG, "hb_hmc_test: direction %d (+1 or -1)", direction);
+  HB_CHECK(p && site_stride >= c->g.vh && (eta1 || direction < 0), HB_E_ARG, "hb_hmc_test: bad arguments");
+  HB_HIP(hipSetDevice(c->device));
+  const int vh = c->g.vh;
+  int rc = 0;
+  double *pd = static_cast<double *>(scratch(c, SLOT_MOM, sizeof(double) * 64 * (size_t)vh, &rc)); HB_TRY(rc);
+  double2 *e1 = nullptr, *e2 = nullptr;
+  HB_TRY(gen_upload(c, pd, p, site_stride));
+  if (direction > 0) {
+    e1 = spinor_scratch(c, SLOT_X, &rc); HB_TRY(rc);
+    HB_TRY(spinor_upload(c, e1, eta1));
+    if (eta2) {
+      e2 = spinor_scratch(c, SLOT_B, &rc); HB_TRY(rc);
+      HB_TRY(spinor_upload(c, e2, eta2));
+    }
+  }
+  // hmc_test.F90:41-47: force-accepted in both directions
+  HB_TRY(trajectory(c, prm, pd, reinterpret_cast<double *>(e1), reinterpret_cast<double *>(e2), -1.0, out,
+                    direction, pd));
+  return gen_download(c, p, pd, site_stride);
 }
 
 int hb_gaussian_d(hb_context *ctx, unsigned long long seed, double var, double *x_d, long long n) {

@@ -170,6 +170,18 @@ void *scratch(Context *c, int slot, size_t bytes, int *rc);
 
 constexpr int kBlock = 256;
 
+// Kernels address every field through buffer resources: a 32-bit byte offset
+// per lane plus a 32-bit plane offset, so one buffer spans less than 4 GiB.
+constexpr unsigned long long kBufferLimit = 1ULL << 32;
+inline bool fits_buffer(unsigned long long bytes) { return bytes < kBufferLimit; }
+// bytes of one stored complex number of a multi-RHS block field
+inline int block_complex_bytes(int precision) { return precision == 32 ? 8 : 4; }
+// a multi-RHS block field holds 12 complex numbers per (site, RHS) in one buffer
+inline bool multi_block_fits(long long vh, int nr, int precision) {
+  return fits_buffer(12ULL * (unsigned long long)vh * (unsigned long long)nr *
+                     (unsigned long long)block_complex_bytes(precision));
+}
+
 // The hardware hands workgroup b to XCD (b mod 8).  This bijection of
 // [0, nblk) gives each of the 8 XCDs (each with its own L2) one contiguous
 // range of sites instead of every 8th block, so that neighbour reuse between

@@ -528,6 +528,10 @@ int multi_fields(Context *c, const MultiPlan &plan, MultiFields<S, G> &f) {
   HB_HIP(hipGetLastError());
   f.U = U;
   if (!plan.clover) return HB_OK;
+  // the block kernels address the clover words of all kappa groups through one buffer
+  HB_CHECK(fits_buffer(72ULL * vh * plan.ngroups * st_bytes<G>()), HB_E_ARG,
+           "cg_multi: the clover terms of %d kappa values at %zu sites per parity exceed the 4 GiB "
+           "span of one buffer", plan.ngroups, vh);
   for (int w = 0; w < 2; w++) {
     f.C[w] = static_cast<typename G::ST *>(scratch(c, SLOT_MULTI + 21 + w, st_bytes<G>() * 72 * vh * plan.ngroups, &rc));
     HB_TRY(rc);
@@ -760,6 +764,7 @@ int cg_multi_prec(Context *c, MultiPlan &plan, int op, int nrhs, const double *k
       double best = 1e30;
       for (int nr : cand) {
         if (nr > nrhs && nr != 1) break;
+        if (!multi_block_fits(c->g.vh, nr, prec)) break;  // larger blocks do not fit either
         for (int tl : tiles) {
           c->multi_tile = tl;
           double ms = 0.0;
@@ -791,6 +796,10 @@ int cg_multi_prec(Context *c, MultiPlan &plan, int op, int nrhs, const double *k
   }
   HB_CHECK(block == 1 || block == 2 || block == 4 || block == 8 || block == 12, HB_E_ARG,
            "cg_multi: block size %d (use 0, 1, 2, 4, 8 or 12)", block);
+  HB_CHECK(multi_block_fits(c->g.vh, block, prec), HB_E_ARG,
+           "cg_multi: a block of %d right-hand sides at %d sites per parity exceeds the 4 GiB span "
+           "of one buffer in precision %d (largest block: %d)", block, c->g.vh, prec,
+           hb_multi_block_limit(c->g.vh, prec));
   c->last_block = block;
   c->last_tile = tile;
   c->multi_tile = tile;
@@ -829,14 +838,15 @@ int cg_multi(Context *c, int op, int nrhs, const double *kappa, const double *rh
     }
     plan.group_of.push_back(gsel);
   }
-  HB_CHECK(plan.ngroups <= 16, HB_E_ARG, "cg_multi: at most 16 distinct kappa values");
+  HB_CHECK(plan.ngroups <= kMaxKappaGroups, HB_E_ARG, "cg_multi: at most %d distinct kappa values",
+           kMaxKappaGroups);
   int rc = 0;
   if (csw > 0.0) {
     plan.clover = true;
     for (int gi = 0; gi < plan.ngroups; gi++) {
-      double2 *A = static_cast<double2 *>(scratch(c, SLOT_MULTI + 30 + 2 * gi, sizeof(double2) * 72 * (size_t)c->g.vh, &rc)); HB_TRY(rc);
-      double2 *Ai = static_cast<double2 *>(scratch(c, SLOT_MULTI + 31 + 2 * gi, sizeof(double2) * 72 * (size_t)c->g.vh, &rc)); HB_TRY(rc);
-      double2 *B = static_cast<double2 *>(scratch(c, SLOT_MULTI + 29, sizeof(double2) * 72 * (size_t)c->g.vh, &rc)); HB_TRY(rc);
+      double2 *A = static_cast<double2 *>(scratch(c, SLOT_MULTI_CLOVER + 2 * gi, sizeof(double2) * 72 * (size_t)c->g.vh, &rc)); HB_TRY(rc);
+      double2 *Ai = static_cast<double2 *>(scratch(c, SLOT_MULTI_CLOVER + 2 * gi + 1, sizeof(double2) * 72 * (size_t)c->g.vh, &rc)); HB_TRY(rc);
+      double2 *B = static_cast<double2 *>(scratch(c, SLOT_MULTI_CLOVER_B, sizeof(double2) * 72 * (size_t)c->g.vh, &rc)); HB_TRY(rc);
       HB_TRY(build_clover_into(c, csw * plan.gkappa[gi], A, Ai, B));
       plan.clovA.push_back(A);
       plan.clovI.push_back(Ai);

@@ -36,8 +36,33 @@ enum {
   SLOT_FORCE3,
   SLOT_FORCE4,
   SLOT_MOM,
-  SLOT_MULTI = 64  // multi-RHS fields start here
+  SLOT_SINGLE_END
 };
+
+// limits that size the slot ranges below
+constexpr int kMaxKappaGroups = 16;  // distinct kappa values of one multi-RHS solve
+constexpr int kGaugeBackups = 4;     // hb_gauge_backup / hb_gauge_restore slots
+constexpr int kMaxMre = 16;          // chronological guess vectors per pseudofermion
+
+// Slot ranges after the single-RHS slots.  Every range is disjoint from the
+// others (checked below), so no solver can overwrite another's fields or a
+// caller's gauge backup.
+enum {
+  SLOT_MULTI = 64,                                  // multi-RHS block fields: +0 .. +24
+  SLOT_MULTI_LAST = SLOT_MULTI + 24,
+  SLOT_MULTI_CLOVER_B = SLOT_MULTI + 29,            // LDL^+ scratch of the per-kappa clover build
+  SLOT_MULTI_CLOVER = SLOT_MULTI + 30,              // per kappa group gi: A/2 at +2 gi, A^-1/2 at +2 gi + 1
+  SLOT_BACKUP = SLOT_MULTI_CLOVER + 2 * kMaxKappaGroups,  // caller-visible gauge backups
+  SLOT_HMC = SLOT_BACKUP + kGaugeBackups,           // HMC fields (hb_hmc.hip), 24 slots
+  SLOT_HMC_MV = SLOT_HMC + 24,                      // kMaxMre products M v_i
+  SLOT_HMC_MRE = SLOT_HMC_MV + kMaxMre,             // 2 bases of kMaxMre vectors
+  SLOT_END = SLOT_HMC_MRE + 2 * kMaxMre
+};
+static_assert(SLOT_SINGLE_END <= SLOT_MULTI, "single-RHS slots overlap the multi-RHS range");
+static_assert(SLOT_MULTI_LAST < SLOT_MULTI_CLOVER_B, "multi-RHS fields overlap the clover scratch");
+static_assert(SLOT_MULTI_CLOVER_B < SLOT_MULTI_CLOVER, "clover scratch overlaps the per-kappa fields");
+static_assert(SLOT_MULTI_CLOVER + 2 * kMaxKappaGroups <= SLOT_BACKUP, "per-kappa clover overlaps the backups");
+static_assert(SLOT_BACKUP + kGaugeBackups <= SLOT_HMC, "gauge backups overlap the HMC fields");
 
 bool use_clover(const Context *c);
 int check_ready(const Context *c);

@@ -33,7 +33,7 @@ module bqcd_hip
      integer(c_int)         :: integrator
      real(c_double)         :: lambda
      real(c_float)          :: rest
-     integer(c_int)         :: maxiter, precision
+     integer(c_int)         :: maxiter, log, precision
      real(c_double)         :: delta
      integer(c_int)         :: max_inner, mre_vectors, batch, verbose
   end type hb_hmc_params
@@ -178,6 +178,10 @@ module bqcd_hip
        import :: c_int, c_ptr
        type(c_ptr), value :: ctx
      end function
+     integer(c_int) function hb_multi_block_limit(volh, precision) bind(C, name="hb_multi_block_limit")
+       import :: c_int
+       integer(c_int), value :: volh, precision
+     end function
      integer(c_int) function hb_multi_tile(ctx) bind(C, name="hb_multi_tile")
        import :: c_int, c_ptr
        type(c_ptr), value :: ctx
@@ -257,6 +261,20 @@ module bqcd_hip
        type(*), intent(in)           :: eta1(*)
        type(*), intent(in), optional :: eta2(*)
        real(c_double), value         :: accept_random
+       type(hb_hmc_out), intent(out) :: out
+     end function
+
+     ! hmc() in its reversibility-test modes (hmc.F90:124-210); p in and out
+     integer(c_int) function hb_hmc_test(ctx, prm, direction, p, site_stride, eta1, eta2, out) &
+          bind(C, name="hb_hmc_test")
+       import :: c_int, c_ptr, hb_hmc_params, hb_hmc_out
+       type(c_ptr), value            :: ctx
+       type(hb_hmc_params), intent(in) :: prm
+       integer(c_int), value         :: direction
+       type(*), intent(inout)        :: p(*)
+       integer(c_int), value         :: site_stride
+       type(*), intent(in), optional :: eta1(*)
+       type(*), intent(in), optional :: eta2(*)
        type(hb_hmc_out), intent(out) :: out
      end function
 

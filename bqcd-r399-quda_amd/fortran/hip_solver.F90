@@ -290,6 +290,18 @@ subroutine hip_hmc(para, conf, out, force_accept, test)
   ! number likewise after them, so with the same seed this reproduces hmc();
   ! the leapfrog and force groups follow para%model.  conf%u and the clover
   ! term come back updated (or unchanged on rejection); conf%phi is not set.
+  !
+  ! test = HMC_TEST_FORWARDS / HMC_TEST_BACKWARDS are hmc()'s reversibility
+  ! modes as hmc_test (hmc_test.F90:28-50) drives them: the forward call draws
+  ! and integrates as usual; the backward call (para%tau negated by the
+  ! caller) draws nothing and integrates from the final momenta of the forward
+  ! call (kept in the saved momentum field, as hmc() keeps its p) with the
+  ! pseudofermion fields of that call; both are force-accepted and both report
+  ! through hmc_test_report, as hmc() does.
+  !
+  ! cg_para%log is honoured (2: solves that reach maxiter are accepted, as in
+  ! BQCD's benchmark inputs), and the trajectory's solves are entered in
+  ! cg_stat, cg_iterations_total and the CG timing bin as cg() would.
 
   use module_hip_solver
   use module_cg
@@ -298,6 +310,9 @@ subroutine hip_hmc(para, conf, out, force_accept, test)
   use module_switches
   use module_vol
   use typedef_hmc
+#ifdef TIMING
+  use module_timing_meas
+#endif
   implicit none
 
   type(hmc_para), intent(in)    :: para
@@ -306,16 +321,21 @@ subroutine hip_hmc(para, conf, out, force_accept, test)
   integer,        intent(in)    :: force_accept
   integer,        intent(in)    :: test
 
-  P_GENERATOR_FIELD, save :: p
+  P_GENERATOR_FIELD, save :: p, p_start
+  P_GAUGE_FIELD, save     :: u_start
   P_SPINCOL_FIELD, save   :: eta1, eta2
   type(hb_hmc_params)     :: prm
   type(hb_hmc_out)        :: o
   real(c_double)          :: r
+  integer(c_int)          :: rc
   REAL, external          :: ranf
 
-  if (test /= 0) call die("hip_hmc(): reversibility tests run with hmc()")
+  TIMING_START(timing_bin_hmc)
+
   if (para%h /= ZERO) call die("hip_hmc(): h /= 0 is not supported")
   if (.not. c_associated(hip_ctx)) call die("hip_hmc(): init_hip_solver() not called")
+  if (test /= HMC_TEST_NONE .and. force_accept == 0) &
+       call die("hip_hmc(): a reversibility test is force-accepted (hmc_test.F90:41-47)")
 
   if (.not. associated(p)) then
      call allocate_gen_field(p)
@@ -323,9 +343,11 @@ subroutine hip_hmc(para, conf, out, force_accept, test)
      call allocate_sc_field(eta2)
   endif
 
-  call hmc_init_p(p)
-  call ran_gauss_volh(NDIRAC * NCOL, eta1, HALF, EVEN)
-  if (switches%hasenbusch) call ran_gauss_volh(NDIRAC * NCOL, eta2, HALF, EVEN)
+  if (test /= HMC_TEST_BACKWARDS) then
+     call hmc_init_p(p)
+     call ran_gauss_volh(NDIRAC * NCOL, eta1, HALF, EVEN)
+     if (switches%hasenbusch) call ran_gauss_volh(NDIRAC * NCOL, eta2, HALF, EVEN)
+  endif
 
   r = -1.0_c_double
   if (force_accept == 0) r = ranf()
@@ -343,6 +365,7 @@ subroutine hip_hmc(para, conf, out, force_accept, test)
   prm%lambda = 0.0_c_double
   prm%rest = real(cg_para%rest, c_float)
   prm%maxiter = cg_para%maxiter
+  prm%log = cg_para%log
   prm%precision = hip_precision
   prm%delta = hip_delta
   prm%max_inner = hip_max_inner
@@ -351,17 +374,44 @@ subroutine hip_hmc(para, conf, out, force_accept, test)
   prm%verbose = 0
 
   call hip_check(hb_load_gauge(hip_ctx, conf%u, int(volh_tot, c_int), 0_c_int), "hb_load_gauge")
-  if (switches%hasenbusch) then
-     call hip_check(hb_hmc_trajectory(hip_ctx, prm, p, int(volh_tot, c_int), eta1, eta2, r, o), "hip_hmc()")
+  if (test == HMC_TEST_NONE) then
+     if (switches%hasenbusch) then
+        rc = hb_hmc_trajectory(hip_ctx, prm, p, int(volh_tot, c_int), eta1, eta2, r, o)
+     else
+        rc = hb_hmc_trajectory(hip_ctx, prm, p, int(volh_tot, c_int), eta1, accept_random=r, out=o)
+     endif
+  else if (test == HMC_TEST_FORWARDS) then
+     if (.not. associated(p_start)) then
+        call allocate_gen_field(p_start)
+        call allocate_g_field(u_start)
+     endif
+     p_start = p
+     u_start = conf%u
+     if (switches%hasenbusch) then
+        rc = hb_hmc_test(hip_ctx, prm, 1_c_int, p, int(volh_tot, c_int), eta1, eta2, o)
+     else
+        rc = hb_hmc_test(hip_ctx, prm, 1_c_int, p, int(volh_tot, c_int), eta1, out=o)
+     endif
   else
-     call hip_check(hb_hmc_trajectory(hip_ctx, prm, p, int(volh_tot, c_int), eta1, accept_random=r, out=o), &
-                    "hip_hmc()")
+     ! hmc_test negated para%tau; the library negates the step of a backward call itself
+     prm%tau = -para%tau
+     rc = hb_hmc_test(hip_ctx, prm, -1_c_int, p, int(volh_tot, c_int), out=o)
   endif
+  call hip_check(rc, "hip_hmc()")
 
   call hip_check(hb_save_gauge(hip_ctx, conf%u, int(volh_tot, c_int), 0_c_int), "hb_save_gauge")
   call xbound_g_field(conf%u)
   if (switches%clover) then
      call hip_check(hb_save_clover(hip_ctx, conf%a, conf%i, conf%b), "hb_save_clover")
+  endif
+
+  ! hmc.F90:145-147 (H with beta), hmc.F90:131-134; the report of hmc_test.F90
+  if (test == HMC_TEST_FORWARDS) then
+     call hmc_test_report(test, p_start, u_start, o%sp_old, o%sg_old * para%beta, &
+                          o%sf1_old, o%sf2_old, o%sd_old)
+  else if (test == HMC_TEST_BACKWARDS) then
+     call hmc_test_report(test, p, conf%u, o%sp_new, o%sg_new * para%beta, &
+                          o%sf1_new, o%sf2_new, o%sd_new)
   endif
 
   out%exp_dh = o%exp_dh
@@ -371,4 +421,16 @@ subroutine hip_hmc(para, conf, out, force_accept, test)
   out%cg_ncall = o%cg_ncall
   out%cg_niter_max = o%cg_niter_max
   out%cg_niter_tot = o%cg_niter_tot
+
+  ! cg_stat as hmc() leaves it (init_cg_stat at its start, cg.F90:126-131 per
+  ! solve), and the running totals of BQCD's timing report (timing.F90:278-285)
+  cg_stat%ncall = o%cg_ncall
+  cg_stat%niter_max = o%cg_niter_max
+  cg_stat%niter_tot = o%cg_niter_tot
+  cg_iterations_total = cg_iterations_total + o%cg_niter_tot
+#ifdef TIMING
+  meas(timing_bin_cg)%n_call = meas(timing_bin_cg)%n_call + o%cg_ncall
+#endif
+
+  TIMING_STOP(timing_bin_hmc)
 end

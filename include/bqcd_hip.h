@@ -139,6 +139,12 @@ int hb_cg_multi(hb_context *ctx, int op, int nrhs, const double *kappa, const do
                 int maxiter, int log, int precision, int block, int *iters, double *true_rtr);
 /* block size chosen by the last hb_cg_multi call */
 int hb_multi_block_size(hb_context *ctx);
+/* largest RHS block size (12, 8, 4, 2 or 1) whose block fields fit the 4 GiB
+ * span of one buffer at volh sites per parity in precision HB_PREC_F32 or
+ * HB_PREC_F16; 0 if none does or the arguments are invalid.  hb_cg_multi
+ * rejects larger explicit blocks and its autotuner skips them.  Host-only (no
+ * device or context needed); no counterpart in the reference. */
+int hb_multi_block_limit(int volh, int precision);
 /* workgroup size (threads) of the block kernels of the last hb_cg_multi call */
 int hb_multi_tile(hb_context *ctx);
 /* workgroup size for hb_cg_multi calls with an explicit block size: 128 or 256
@@ -217,6 +223,10 @@ typedef struct {
   double lambda;                  /* Omelyan parameter (0 = 0.1931833275037836) */
   float rest;                     /* cg_para%rest (absolute |r|^2) */
   int maxiter;                    /* cg_para%maxiter */
+  int log;                        /* cg_para%log: 2 accepts a solve that reaches maxiter
+                                     without convergence (cg.F90:119-124, BQCD's
+                                     solver_ignore_no_convergence 2); otherwise such a
+                                     solve ends the trajectory with HB_E_NOCONV */
   int precision;                  /* solver: HB_PREC_F64 (cg.F90), HB_PREC_F32, HB_PREC_F16 */
   double delta;                   /* reliable-update threshold of the mixed solver */
   int max_inner;
@@ -264,6 +274,12 @@ int hb_hmc_trajectory_d(hb_context *ctx, const hb_hmc_params *prm, const double 
  * backward h_new to the forward h_old, up to the solver tolerance. */
 int hb_hmc_test_d(hb_context *ctx, const hb_hmc_params *prm, int direction, double *p_d,
                   const double *eta1_d, const double *eta2_d, hb_hmc_out *out);
+/* the same on host arrays in BQCD layouts (p with site_stride; eta1/eta2 are
+ * read for direction +1 only): p holds the momenta on entry and the final
+ * momenta on return, as hmc()'s saved momentum field between the forward and
+ * the backward call of hmc_test (hmc_test.F90:41-47). */
+int hb_hmc_test(hb_context *ctx, const hb_hmc_params *prm, int direction, double *p, int site_stride,
+                const double *eta1, const double *eta2, hb_hmc_out *out);
 /* ---- BQCD configuration files (host only, no device needed) -----------
  * conf_write / conf_read (conf.F90:305-470) with field_io (comm/field_io_mpi.F90)
  * and the ConfInfoHeader / CheckSum blocks (conf_info.F90): one big-endian file

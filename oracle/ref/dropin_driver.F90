@@ -9,8 +9,12 @@
 ! stdin (as golden_driver): Lx Ly Lz Lt / kappa csw rho / rest maxiter /
 ! step beta / input directory (u.bin, eta_e.bin, mom.bin) / (unused) / 0
 !
-! The last comparison is a whole trajectory: hmc() against hip_hmc() after
-! the same ranset (model C with rho, model A without).
+! The last comparisons are whole trajectories: hmc() against hip_hmc() after
+! the same ranset (model C with rho, model A without), then the same with
+! every CG stopped after a few iterations (cg_para%log = 2, the reference's
+! benchmark mode), then hmc_test's forward / backward pair
+! (HMC_TEST_FORWARDS, then HMC_TEST_BACKWARDS with tau negated) with each.
+! BQCD's CG statistics (cg_stat, cg_iterations_total) must advance alike.
 !
 ! Prints one "DROPIN name value" line per comparison (iteration counts, and
 ! max |hip - reference| / max |reference| of each result) and exits with
@@ -36,6 +40,7 @@ program dropin_driver
   type(hmc_conf)  :: conf, conf2
   type(hmc_out)   :: out_ref, out_hip
   integer         :: lat(4), maxiter, bench_iters, it_ref, it_hip, mu, eo, nfail
+  integer         :: tot_ref, tot_hip
   REAL            :: kappa, csw, rho, step, beta
   real            :: rest
   character(len=512) :: indir, outdir
@@ -190,16 +195,47 @@ program dropin_driver
   call copy_conf(conf2, conf)
   call init_cg_stat()
   call ranset(4711_8, 0_8)
+  tot_ref = cg_iterations_total
   call hmc(hp, conf, out_ref, 1, 0)
+  tot_ref = cg_iterations_total - tot_ref
   call ranset(4711_8, 0_8)
+  tot_hip = cg_iterations_total
   call hip_hmc(hp, conf2, out_hip, 1, 0)
-  call report_int('hmc_cg_iters_ref', out_ref%cg_niter_tot)
-  call report_int('hmc_cg_iters_hip', out_hip%cg_niter_tot)
+  tot_hip = cg_iterations_total - tot_hip
+  call compare_traj('hmc', 1.0e-9_8, 1.0e-10_8)
+  call report_int('hmc_cg_total_ref', tot_ref)
+  call report_int('hmc_cg_total_hip', tot_hip)
+  if (tot_ref /= tot_hip .or. tot_ref /= out_ref%cg_niter_tot) nfail = nfail + 1
+
+  ! --- the benchmark mode: every CG stopped after 12 iterations (log = 2) ----
+  call init_cg_para(0.0, 12, 2)
+  call copy_conf(conf2, conf)
+  call ranset(4712_8, 0_8)
+  call hmc(hp, conf, out_ref, 1, 0)
+  call ranset(4712_8, 0_8)
+  call hip_hmc(hp, conf2, out_hip, 1, 0)
+  call compare_traj('hmc_fixed12', 1.0e-9_8, 1.0e-10_8)
+  if (out_hip%cg_niter_max /= 12 .or. out_hip%cg_niter_tot /= 12 * out_hip%cg_ncall) nfail = nfail + 1
+  call init_cg_para(rest, maxiter, 0)
+
+  ! --- hmc_test.F90:28-50: forward, then backward with tau negated -----------
+  call copy_conf(conf2, conf)
+  call ranset(4713_8, 0_8)
+  call hmc(hp, conf, out_ref, 1, HMC_TEST_FORWARDS)
+  call ranset(4713_8, 0_8)
+  call hip_hmc(hp, conf2, out_hip, 1, HMC_TEST_FORWARDS)
+  call compare_traj('hmc_test_forward', 1.0e-9_8, 1.0e-10_8)
+  hp%tau = -hp%tau
+  call hmc(hp, conf, out_ref, 1, HMC_TEST_BACKWARDS)
+  call hip_hmc(hp, conf2, out_hip, 1, HMC_TEST_BACKWARDS)
+  hp%tau = -hp%tau
+  call report_real('hmc_test_backward_sg_ref', out_ref%sg)
+  call report_real('hmc_test_backward_sg_hip', out_hip%sg)
+  if (.not. abs(out_hip%sg - out_ref%sg) <= 1.0e-10_8 * abs(out_ref%sg)) nfail = nfail + 1
+  call report_int('hmc_test_backward_cg_iters_ref', out_ref%cg_niter_tot)
+  call report_int('hmc_test_backward_cg_iters_hip', out_hip%cg_niter_tot)
   if (out_ref%cg_niter_tot /= out_hip%cg_niter_tot) nfail = nfail + 1
-  call report_real('hmc_exp_dh_ref', out_ref%exp_dh)
-  call report_real('hmc_exp_dh_hip', out_hip%exp_dh)
-  if (.not. abs(out_hip%exp_dh - out_ref%exp_dh) <= 1.0e-9_8 * abs(out_ref%exp_dh)) nfail = nfail + 1
-  call report(conf2%u, conf%u, 2 * size(conf%u), 'hmc_links', 1.0e-10_8)
+  call report(conf2%u, conf%u, 2 * size(conf%u), 'hmc_test_backward_links', 1.0e-10_8)
 
   call end_hip_solver()
 
@@ -207,6 +243,20 @@ program dropin_driver
   if (nfail /= 0) stop 1
 
 CONTAINS
+
+  ! a trajectory of hip_hmc (conf2, out_hip) against hmc() (conf, out_ref)
+  subroutine compare_traj(name, tol_dh, tol_u)
+    character(len=*) :: name
+    real(8) :: tol_dh, tol_u
+    call report_int(name // '_cg_iters_ref', out_ref%cg_niter_tot)
+    call report_int(name // '_cg_iters_hip', out_hip%cg_niter_tot)
+    if (out_ref%cg_niter_tot /= out_hip%cg_niter_tot) nfail = nfail + 1
+    if (out_ref%cg_ncall /= out_hip%cg_ncall) nfail = nfail + 1
+    call report_real(name // '_exp_dh_ref', out_ref%exp_dh)
+    call report_real(name // '_exp_dh_hip', out_hip%exp_dh)
+    if (.not. abs(out_hip%exp_dh - out_ref%exp_dh) <= tol_dh * abs(out_ref%exp_dh)) nfail = nfail + 1
+    call report(conf2%u, conf%u, 2 * size(conf%u), name // '_links', tol_u)
+  end subroutine compare_traj
 
   subroutine copy_conf(dst, src)
     type(hmc_conf) :: dst, src

@@ -10,7 +10,8 @@
 !   Lx Ly Lz Lt
 !   beta kappa csw rho
 !   tau ntau m_scale model        (tau = trajectory length / ntau, as hmc_para)
-!   rest maxiter mre_vectors
+!   rest maxiter mre_vectors log   (log = cg_para%log: 2 accepts solves that
+!                                   reach maxiter, cg.F90:119-124)
 !   seed                           (ranset; 0 = the default seed of ranf.F90)
 !   input directory                (u.bin: physical links, complex(3,3,volh,0:1,4))
 !   output directory               (info.txt, u_new.bin)
@@ -33,7 +34,7 @@ program traj_driver
   type(hmc_para)  :: hp
   type(hmc_conf)  :: conf
   type(hmc_out)   :: out
-  integer         :: lat(4), maxiter, ntau, m_scale, nmre, mu, eo
+  integer         :: lat(4), maxiter, ntau, m_scale, nmre, cglog, mu, eo
   integer(8)      :: seed
   REAL            :: kappa, csw, rho, tau, beta, plaq
   real            :: rest
@@ -45,7 +46,7 @@ program traj_driver
   read(*,*) lat
   read(*,*) beta, kappa, csw, rho
   read(*,*) tau, ntau, m_scale, model
-  read(*,*) rest, maxiter, nmre
+  read(*,*) rest, maxiter, nmre, cglog
   read(*,*) seed
   read(*,'(a)') indir
   read(*,'(a)') outdir
@@ -59,7 +60,7 @@ program traj_driver
   call init_modules()
   call init_flip_bc()
   call init_xbound()
-  call init_cg_para(rest, maxiter, 0)
+  call init_cg_para(rest, maxiter, cglog)
   call init_cg_stat()
   mre_n_vec = nmre
 
@@ -115,6 +116,8 @@ program traj_driver
   write(20, '(a, i10)') 'cg_ncall ', out%cg_ncall
   write(20, '(a, i10)') 'cg_niter_tot ', out%cg_niter_tot
   write(20, '(a, i10)') 'cg_niter_max ', out%cg_niter_max
+  call ranget(seed)
+  write(20, '(a, i20)') 'seed_after ', seed
   close(20)
   do mu = 1, DIM
      do eo = EVEN, ODD

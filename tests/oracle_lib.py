@@ -130,10 +130,13 @@ class Lattice:
 
     def hmc_draws(self, seed, hasenbusch=True):
         """the random fields of one trajectory of hmc.F90 in its order of draws
-        after ranset(seed): momenta (hmc_init_p.F90: 8 ran_gauss_volh(4, ., 1)
-        for mu, eo), then the pseudofermion noise eta1 and, with Hasenbusch,
-        eta2 (hmc_init_phi.F90: ran_gauss_volh(12, ., 1/2, EVEN))"""
-        lib().or_ranset(seed, 0)
+        after ranset(seed) (seed None: continuing the current stream, as the
+        next trajectory of a run): momenta (hmc_init_p.F90: 8
+        ran_gauss_volh(4, ., 1) for mu, eo), then the pseudofermion noise eta1
+        and, with Hasenbusch, eta2 (hmc_init_phi.F90: ran_gauss_volh(12, ., 1/2,
+        EVEN))"""
+        if seed is not None:
+            lib().or_ranset(seed, 0)
         p = np.zeros((4, 2, self.volh, 8))
         for mu in range(4):
             for eo in range(2):

@@ -200,3 +200,24 @@ def test_ctypes_argtypes_match_the_header():
         assert protos.get(m.group(1)) == n, (m.group(1), n, protos.get(m.group(1)))
         seen += 1
     assert seen >= len(protos) - 1   # all but the argument-less hb_last_error
+
+
+@pytest.mark.skipif(not os.path.exists(LIB), reason="libbqcd_hip.so not built")
+def test_multi_block_limit_keeps_buffers_below_4gib():
+    """hb_multi_block_limit (host only, no device): the largest RHS block whose
+    block spinor fields (12 complex numbers per site and RHS, 8 bytes each in
+    FP32, 4 in 16 bits) stay below the 4 GiB span of one buffer"""
+    import bqcd_hip
+
+    vh = {"24x48": 24 ** 3 * 48 // 2, "32x64": 32 ** 3 * 64 // 2, "48x96": 48 ** 3 * 96 // 2,
+          "64x128": 64 ** 3 * 128 // 2}
+    lim = bqcd_hip.multi_block_limit
+    assert lim(vh["24x48"], "f32") == 12 and lim(vh["24x48"], "f16") == 12
+    assert lim(vh["32x64"], "f32") == 12 and lim(vh["32x64"], "f16") == 12
+    assert lim(vh["48x96"], "f32") == 8 and lim(vh["48x96"], "f16") == 12   # 6.1 GB for 12 in FP32
+    assert lim(vh["64x128"], "f32") == 2 and lim(vh["64x128"], "f16") == 4
+    for p in ("f32", "f16"):
+        for k, v in vh.items():
+            nr = lim(v, p)
+            assert 12 * v * nr * (8 if p == "f32" else 4) < 2 ** 32, (k, p)
+    assert lim(0, "f16") == 0 and lim(vh["24x48"], "f64") == 0

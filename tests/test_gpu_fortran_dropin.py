@@ -3,10 +3,16 @@ from BQCD itself: oracle/_ref/bqcd_dropin links the reference CPU path with
 libbqcd_hip.so and runs, in one process on the same fields, each reference
 routine next to the drop-in that replaces it (cg / hip_solver on M~+M~ and
 W+W, dsf / hip_dsf, dsd / hip_dsd, dsg / hip_dsg, hmc_u / hip_hmc_u with the
-clover rebuild).  The binary reports every comparison and fails on:
-different CG iteration counts, solutions apart by more than 1e-9 of their
-maximum (both stop at the same |r|^2 bound), forces and updated fields apart
-by more than 1e-12, or links left in a different boundary-flip state.
+clover rebuild), then whole trajectories: hmc() against hip_hmc() from the
+same seed; the same with every CG stopped after 12 iterations (cg_para%log
+= 2, BQCD's benchmark mode); and hmc_test's forward / backward pair
+(HMC_TEST_FORWARDS, then HMC_TEST_BACKWARDS with tau negated).  The binary
+reports every comparison and fails on: different CG iteration counts or
+calls, solutions apart by more than 1e-9 of their maximum (both stop at the
+same |r|^2 bound), forces and updated fields apart by more than 1e-12,
+exp(-Delta H) apart by 1e-9 relative, links after a trajectory apart by 1e-10,
+links left in a different boundary-flip state, or BQCD's running CG
+iteration total (cg_iterations_total) advancing differently.
 
 The binary is built in the build container (make -C oracle/ref dropin);
 without it the test is skipped."""
@@ -43,7 +49,7 @@ def test_fortran_dropin_against_reference(case):
         ctl = "%d %d %d %d\n%.17g %.17g %.17g\n%.9g %d\n%.17g %.17g\n%s\n%s\n0\n" % (
             *L, kappa, csw, rho, rest, maxiter, step, beta, d, d)
         print("bqcd_dropin", case, flush=True)
-        r = subprocess.run([BIN], input=ctl.encode(), capture_output=True, timeout=120)
+        r = subprocess.run([BIN], input=ctl.encode(), capture_output=True, timeout=120, cwd=d)
     out = r.stdout.decode()
     print(out)
     print(r.stderr.decode()[-2000:])
@@ -53,3 +59,7 @@ def test_fortran_dropin_against_reference(case):
     assert lines["mtdagmt_iters_ref"] == lines["mtdagmt_iters_hip"]
     if rho:
         assert lines["wdw_iters_ref"] == lines["wdw_iters_hip"]
+    for name in ("hmc", "hmc_fixed12", "hmc_test_forward"):
+        assert lines[name + "_cg_iters_ref"] == lines[name + "_cg_iters_hip"]
+    assert lines["hmc_cg_total_ref"] == lines["hmc_cg_total_hip"]
+    assert lines["hmc_test_backward_cg_iters_ref"] == lines["hmc_test_backward_cg_iters_hip"]

@@ -75,6 +75,87 @@ def test_trajectory_vs_reference(hip, golden, case):
     g.close()
 
 
+def benchmark_params(hip, ntau, tau):
+    """work/input.smp and work/input.mpp: beta 5, kappa 0.13, csw 1.923076923076923,
+    model A (rho 0), leapfrog with m_scale 1, solver_rest 1e-99 (0 as float32),
+    solver_maxiter 100, solver_ignore_no_convergence 2, no MRE"""
+    return hip.hmc_params(beta=5.0, kappa=0.13, csw=1.923076923076923, rho=0.0, tau=tau, ntau=ntau,
+                          m_scale=1, model="A", integrator="leapfrog", rest=1e-99, maxiter=100, log=2,
+                          mre_vectors=0)
+
+
+def cold_links(lat):
+    u = lat.gauge()
+    for k in range(3):
+        u[:, :, :, k, k, 0] = 1.0
+    return u
+
+
+def test_reference_benchmark_smp(hip, golden):
+    """The reference's own benchmark input work/input.smp: 24^3x48, cold start,
+    default seed, one force-accepted trajectory of 10 steps of 0.02 with every
+    CG stopped after 100 iterations.  Against the reference hmc() run here
+    (traj_smp): the final plaquette action to 1e-10, the links at 64 sites to
+    1e-9 and the CG counts; and against the reference's printed result
+    (work/out.smp.reference:40): PlaqEnergy 0.2512245607, 11 calls, 1100
+    iterations, at most 100."""
+    f = golden("traj_smp")
+    L = (24, 24, 24, 48)
+    lat = ol.Lattice(L)
+    vol = 2 * lat.volh
+    p, eta1, _ = lat.hmc_draws(0, hasenbusch=False)
+    assert ol.ranget() == int(f["info_seed_after"])
+    g = hip.Lattice(L)
+    g.load_gauge(cold_links(lat))
+    print("smp benchmark trajectory", flush=True)
+    out = g.hmc_trajectory(benchmark_params(hip, 10, 0.02), p, eta1)
+    print(out, out["sg_new"] / (6 * vol), flush=True)
+    assert (out["cg_ncall"], out["cg_niter_tot"], out["cg_niter_max"]) == (11, 1100, 100)
+    assert abs(out["sg_new"] - float(f["info_sg_new"])) < 1e-10 * float(f["info_sg_new"])
+    assert round(out["sg_new"] / (6 * vol), 10) == 0.2512245607
+    u = g.save_gauge()
+    sites = f["sites"]
+    assert np.max(np.abs(u[:, :, sites] - f["u_new_sample"])) < 1e-9
+    g.close()
+
+
+@pytest.mark.timeout(900)
+def test_reference_benchmark_mpp(hip):
+    """The reference's benchmark input work/input.mpp: 48^3x96, cold start,
+    default seed, 20 steps of 0.01, fixed 100-iteration CGs, two trajectories:
+    the first force-accepted, the second a Metropolis step with the next ranf
+    number.  The reference printed (work/out.mpp.reference:41,45,50) PlaqEnergy
+    0.2507903838 after both (the second was rejected with exp(-Delta H) = 0),
+    21 CG calls with 2100 iterations and at most 100 in each, and the final
+    seed 166035151690961."""
+    L = (48, 48, 48, 96)
+    lat = ol.Lattice(L)
+    vol = 2 * lat.volh
+    g = hip.Lattice(L)
+    g.load_gauge(cold_links(lat))
+    prm = benchmark_params(hip, 20, 0.01)
+    print("mpp benchmark trajectory 1 (force accepted)", flush=True)
+    p, eta1, _ = lat.hmc_draws(0, hasenbusch=False)
+    out1 = g.hmc_trajectory(prm, p, eta1)
+    print(out1, out1["sg_new"] / (6 * vol), flush=True)
+    assert (out1["cg_ncall"], out1["cg_niter_tot"], out1["cg_niter_max"]) == (21, 2100, 100)
+    assert round(out1["sg_new"] / (6 * vol), 10) == 0.2507903838
+    print("mpp benchmark trajectory 2", flush=True)
+    p, eta1, _ = lat.hmc_draws(None, hasenbusch=False)
+    del _
+    r = ol.ranf()
+    assert ol.ranget() == 166035151690961
+    out2 = g.hmc_trajectory(prm, p, eta1, accept_random=r)
+    del p, eta1
+    print(out2, flush=True)
+    assert (out2["cg_ncall"], out2["cg_niter_tot"], out2["cg_niter_max"]) == (21, 2100, 100)
+    assert out2["accepted"] == 0 and out2["exp_dh"] < r
+    # rejected: the links (and plaquette) of the first trajectory's end
+    assert round(out2["plaquette"], 12) == round(1.0 - out1["sg_new"] / (6 * vol), 12)
+    assert round(1.0 - out2["plaquette"], 10) == 0.2507903838
+    g.close()
+
+
 def dh_at(hip, g, u0, prm, draws):
     g.load_gauge(u0)
     out = g.hmc_trajectory(prm, *draws)

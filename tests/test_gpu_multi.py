@@ -124,3 +124,42 @@ def test_multi_ragged_lattice_partial_blocks(L, tile):
             check(lat, refs, xs, its, trs, phis, kap, rests, 2)
     finally:
         g.close()
+
+
+def test_sixteen_kappa_groups_leave_gauge_backups_alone():
+    """Regression: the per-kappa clover fields of a multi-RHS solve with the
+    largest number of distinct kappa values (16) and the gauge backups live in
+    disjoint scratch ranges; a backup taken before the solve restores the
+    links bit for bit, and an HMC trajectory (which keeps its own copy of the
+    start links) leaves the caller's backup slots alone too."""
+    import bqcd_hip
+
+    L = (4, 4, 4, 4)
+    lat = ol.Lattice(L)
+    u = lat.gauge_exp_random(556, 0.3)
+    u2 = lat.gauge_exp_random(557, 0.3)
+    g = bqcd_hip.Lattice(L)
+    g.load_gauge(u)
+    for slot in range(4):
+        g.gauge_backup(slot)
+    kappas = [0.10 + 0.001 * j for j in range(16)]
+    phis = [lat.spinor_gauss(950 + j, 0) for j in range(16)]
+    rests = [1e-20 * float(np.sum(p * p)) for p in phis]
+    print("16 kappa groups", flush=True)
+    xs, its, trs, blk = g.cg_multi("mtdagmt", phis, kappas, csw=CSW, rests=rests, precision="f16", block=4)
+    assert all(trs[k] <= rests[k] for k in range(16))
+    g.load_gauge(u2)
+    for slot in range(4):
+        g.gauge_restore(slot)
+        assert np.array_equal(g.save_gauge(), u), slot
+    # a rejected trajectory restores its start from its own copy; slot 0 still holds u
+    g.load_gauge(u2)
+    draws = lat.hmc_draws(80)
+    prm = bqcd_hip.hmc_params(beta=5.29, kappa=0.12, csw=CSW, rho=0.1, tau=0.05, ntau=2, model="C",
+                              rest=1e-22, maxiter=5000)
+    out = g.hmc_trajectory(prm, *draws, accept_random=2.0e300)
+    assert out["accepted"] == 0
+    assert np.array_equal(g.save_gauge(), u2)
+    g.gauge_restore(0)
+    assert np.array_equal(g.save_gauge(), u)
+    g.close()

@@ -150,3 +150,49 @@ def test_golden_trajectory_is_the_reference_quick_test(golden):
     assert int(f["info_cg_ncall"]) == 43
     assert int(f["info_cg_niter_tot"]) == 133
     assert int(f["info_cg_niter_max"]) == 8
+
+
+def test_golden_trajectory_is_the_reference_smp_benchmark(golden):
+    """traj_smp is the reference's hmc() (r399, libd21, OpenMP) run with the
+    parameters of its own benchmark input work/input.smp (24^3x48, cold start,
+    default seed, model A, fixed 100-iteration CGs); it reproduces the
+    reference's printed result, work/out.smp.reference:40,47 (written by an
+    earlier revision with D2 on 40 threads): PlaqEnergy 0.2512245607, 11 CG
+    calls, 1100 iterations, at most 100, and the final ranf seed."""
+    f = golden("traj_smp")
+    vol = 24 ** 3 * 48
+    assert round(float(f["info_sg_new"]) / (6 * vol), 10) == 0.2512245607
+    assert int(f["info_cg_ncall"]) == 11
+    assert int(f["info_cg_niter_tot"]) == 1100
+    assert int(f["info_cg_niter_max"]) == 100
+    assert int(f["info_seed_after"]) == 23583982622317
+
+
+@pytest.mark.parametrize("L,ntraj,accept_draws,seed_after", [
+    ((24, 24, 24, 48), 1, 0, 23583982622317),     # work/out.smp.reference:47
+    ((48, 48, 48, 96), 2, 1, 166035151690961),    # work/out.mpp.reference:50
+])
+def test_benchmark_draw_counts(L, ntraj, accept_draws, seed_after):
+    """The reference's benchmark runs end on the ranf seed they print.  A model
+    A trajectory draws 8 x 4 complex Gaussians per site for the momenta
+    (hmc_init_p) and 12 for the noise (hmc_init_phi), two uniforms each, and
+    a Metropolis step one more uniform: skipping that many numbers from the
+    default seed reproduces the printed seed, which pins the order and number
+    of draws the GPU trajectories are fed with at these sizes."""
+    vh = L[0] * L[1] * L[2] * L[3] // 2
+    ol.ranset(0, ntraj * 88 * vh + accept_draws)
+    assert ol.ranget() == seed_after
+
+
+def test_draws_consume_the_stream_as_the_reference():
+    """drawing the fields of one trajectory (hmc_draws) advances ranf by 88
+    numbers per site, as the skip-ahead above assumes"""
+    lat = ol.Lattice((4, 4, 4, 6))
+    lat.hmc_draws(0, hasenbusch=False)
+    s1 = ol.ranget()
+    ol.ranset(0, 88 * lat.volh)
+    assert ol.ranget() == s1
+    lat.hmc_draws(None, hasenbusch=False)   # the next trajectory continues the stream
+    s2 = ol.ranget()
+    ol.ranset(0, 2 * 88 * lat.volh)
+    assert ol.ranget() == s2
