@@ -113,6 +113,8 @@ def lib():
         L.hb_apply_d.argtypes = [_vp, ctypes.c_int, ctypes.c_double, ctypes.c_double, _vp, _vp]
         L.hb_cg_d.argtypes = [_vp, ctypes.c_int, ctypes.c_double, ctypes.c_double, _vp, _vp,
                               ctypes.c_float, ctypes.c_int, ctypes.c_int, _ip, _dp]
+        L.hb_cg_d_noalloc.argtypes = [_vp, ctypes.c_int, ctypes.c_double, ctypes.c_double, _vp, _vp,
+                                      ctypes.c_float, ctypes.c_int, ctypes.c_int, _ip, _dp]
         L.hb_spinor_to_device.argtypes = [_vp, _vp, _dp]
         L.hb_spinor_to_host.argtypes = [_vp, _dp, _vp]
         L.hb_last_timing.argtypes = [_vp, _dp, _ip]
@@ -467,6 +469,15 @@ class Lattice:
         tr = ctypes.c_double(0.0)
         _check(lib().hb_cg_d(self._h, OP[op], kappa, rho, self._s(x), self._s(b), ctypes.c_float(rest),
                              maxiter, log, ctypes.byref(it), ctypes.byref(tr)))
+        return it.value, tr.value
+
+    def cg_d_noalloc(self, op, x, b, kappa, rho=0.0, rest=1e-20, maxiter=1000, log=0):
+        """cg_d with the library's static work area: allocates no device memory
+        (volh <= HB_NOALLOC_MAX_VOLH)"""
+        it = ctypes.c_int(0)
+        tr = ctypes.c_double(0.0)
+        _check(lib().hb_cg_d_noalloc(self._h, OP[op], kappa, rho, self._s(x), self._s(b),
+                                     ctypes.c_float(rest), maxiter, log, ctypes.byref(it), ctypes.byref(tr)))
         return it.value, tr.value
 
     def cg_mixed_d(self, op, x, b, kappa, rho=0.0, rest=1e-20, maxiter=10000, log=0,

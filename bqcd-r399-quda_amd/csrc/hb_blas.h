@@ -53,10 +53,14 @@ struct FinStore2 {
 
 int reduce_grid(const Context *c, int n);
 int clover_mult_f64(Context *c, int which, int eo, double2 *out, const double2 *in);
-int cg_start(Context *c, double2 *r, double2 *p, const double2 *b, CgState *st);
-int cg_update(Context *c, double2 *x, double2 *r, const double2 *p, const double2 *q, CgState *st);
+// `red`: per-block partial sums of the reductions (at least nblocks(vh)
+// doubles); null selects the context's own buffer
+int cg_init(Context *c, CgState *st, double rest, int maxiter);
+int cg_start(Context *c, double2 *r, double2 *p, const double2 *b, CgState *st, double *red = nullptr);
+int cg_update(Context *c, double2 *x, double2 *r, const double2 *p, const double2 *q, CgState *st,
+              double *red = nullptr);
 int cg_pupdate(Context *c, double2 *p, const double2 *r, const CgState *st);
-int residual(Context *c, double2 *out, const double2 *b, double *res_d);
+int residual(Context *c, double2 *out, const double2 *b, double *res_d, double *red = nullptr);
 int dot_norm(Context *c, const double2 *x, const double2 *y, double *res_d);
 int zero_f64(Context *c, double2 *p, size_t n);
 int copy_f64(Context *c, double2 *d, const double2 *s, size_t n);

@@ -152,18 +152,37 @@ int reduce_grid(const Context *c, int n) {
   return b < cap ? b : cap;
 }
 
-int cg_start(Context *c, double2 *r, double2 *p, const double2 *b, CgState *st) {
-  const int vh = c->g.vh, grid = nblocks(vh);
-  k_cg_start<<<grid, kBlock, 0, c->stream>>>(r, p, b, vh, c->red);
-  k_finalize<1><<<1, kFinThreads, 0, c->stream>>>(c->red, grid, nullptr, FinCgStart{st});
+// every field of the CG state cleared, then the stopping bound and maxiter
+__global__ void k_cg_init(CgState *st, double rest, int maxiter) {
+  if (threadIdx.x == 0) {
+    CgState s{};
+    s.rest = rest;
+    s.maxiter = maxiter;
+    *st = s;
+  }
+}
+
+int cg_init(Context *c, CgState *st, double rest, int maxiter) {
+  k_cg_init<<<1, 64, 0, c->stream>>>(st, rest, maxiter);
   HB_HIP(hipGetLastError());
   return HB_OK;
 }
 
-int cg_update(Context *c, double2 *x, double2 *r, const double2 *p, const double2 *q, CgState *st) {
+int cg_start(Context *c, double2 *r, double2 *p, const double2 *b, CgState *st, double *red) {
   const int vh = c->g.vh, grid = nblocks(vh);
-  k_cg_update<<<grid, kBlock, 0, c->stream>>>(x, r, p, q, vh, st, c->red);
-  k_finalize<1><<<1, kFinThreads, 0, c->stream>>>(c->red, grid, &st->done, FinCgUpdate{st});
+  if (!red) red = c->red;
+  k_cg_start<<<grid, kBlock, 0, c->stream>>>(r, p, b, vh, red);
+  k_finalize<1><<<1, kFinThreads, 0, c->stream>>>(red, grid, nullptr, FinCgStart{st});
+  HB_HIP(hipGetLastError());
+  return HB_OK;
+}
+
+int cg_update(Context *c, double2 *x, double2 *r, const double2 *p, const double2 *q, CgState *st,
+              double *red) {
+  const int vh = c->g.vh, grid = nblocks(vh);
+  if (!red) red = c->red;
+  k_cg_update<<<grid, kBlock, 0, c->stream>>>(x, r, p, q, vh, st, red);
+  k_finalize<1><<<1, kFinThreads, 0, c->stream>>>(red, grid, &st->done, FinCgUpdate{st});
   HB_HIP(hipGetLastError());
   return HB_OK;
 }
@@ -175,10 +194,11 @@ int cg_pupdate(Context *c, double2 *p, const double2 *r, const CgState *st) {
   return HB_OK;
 }
 
-int residual(Context *c, double2 *out, const double2 *b, double *res_d) {
+int residual(Context *c, double2 *out, const double2 *b, double *res_d, double *red) {
   const int vh = c->g.vh, grid = nblocks(vh);
-  k_residual<<<grid, kBlock, 0, c->stream>>>(out, b, vh, c->red);
-  k_finalize<1><<<1, kFinThreads, 0, c->stream>>>(c->red, grid, nullptr, FinStore1{res_d});
+  if (!red) red = c->red;
+  k_residual<<<grid, kBlock, 0, c->stream>>>(out, b, vh, red);
+  k_finalize<1><<<1, kFinThreads, 0, c->stream>>>(red, grid, nullptr, FinStore1{res_d});
   HB_HIP(hipGetLastError());
   return HB_OK;
 }

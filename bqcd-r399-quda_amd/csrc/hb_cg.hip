@@ -26,7 +26,7 @@ DslashArgs<SF64> dslash_args64(Context *c, int out_parity, double2 *out, const d
 // (m_tilde.F90:61-71 / :98-108 with w_mult.F90's sc_axpy folded in).  With
 // dot_out set, also Re<dotv, out> (sc_dot) in the same pass.
 int mtil_f64(Context *c, bool dag, double kappa, double rho, double2 *out, const double2 *in,
-             double2 *t, double *dot_out, const double2 *dotv, const int *skip) {
+             double2 *t, double *dot_out, const double2 *dotv, const int *skip, double *red) {
   const bool clov = use_clover(c);
   DslashArgs<SF64> a1 = dslash_args64(c, 1, t, in);
   a1.C = clov ? c->clov[HB_CLOVER_AINV] : nullptr;
@@ -41,6 +41,7 @@ int mtil_f64(Context *c, bool dag, double kappa, double rho, double2 *out, const
   if (dot_out) {
     a2.dotv = {const_cast<double2 *>(dotv), nullptr};
     a2.dot_out = dot_out;
+    if (red) a2.partials = red;
     HB_TRY(launch_dslash(a2, dag, EPI_MTDOT, c->stream));
   } else {
     HB_TRY(launch_dslash(a2, dag, EPI_MTIL, c->stream));
@@ -49,7 +50,7 @@ int mtil_f64(Context *c, bool dag, double kappa, double rho, double2 *out, const
 }
 
 int apply_f64(Context *c, int op, double kappa, double rho, double2 *out, const double2 *in,
-              double2 *t, double2 *y, double *dot_out, const int *skip) {
+              double2 *t, double2 *y, double *dot_out, const int *skip, double *red) {
   switch (op) {
     case HB_OP_MTIL: return mtil_f64(c, false, kappa, 0.0, out, in, t, nullptr, nullptr, skip);
     case HB_OP_MTIL_DAG: return mtil_f64(c, true, kappa, 0.0, out, in, t, nullptr, nullptr, skip);
@@ -59,7 +60,7 @@ int apply_f64(Context *c, int op, double kappa, double rho, double2 *out, const 
     case HB_OP_WDAGW: {
       double r = op == HB_OP_WDAGW ? rho : 0.0;
       HB_TRY(mtil_f64(c, false, kappa, r, y, in, t, nullptr, nullptr, skip));
-      return mtil_f64(c, true, kappa, r, out, y, t, dot_out, in, skip);
+      return mtil_f64(c, true, kappa, r, out, y, t, dot_out, in, skip, red);
     }
   }
   set_error("unknown operator %d", op);
@@ -78,34 +79,20 @@ double2 *spinor_scratch(Context *c, int slot, int *rc) {
   return static_cast<double2 *>(scratch(c, slot, spinor_bytes(c), rc));
 }
 
-// FP64 CG on device fields.  x: initial guess in, solution out.
-int cg_f64(Context *c, int op, double kappa, double rho, double2 *x, const double2 *b, float rest,
-           int maxiter, int log, int *iters, double *true_rtr) {
+// The FP64 CG iteration on device fields, with its work fields given: x is
+// the initial guess in and the solution out.  `hs` is host memory the state
+// is read back into between chunks of iterations.
+int cg_f64_run(Context *c, const CgWork &w, int op, double kappa, double rho, double2 *x,
+               const double2 *b, float rest, int maxiter, int log, int *iters, double *true_rtr,
+               int *n_matvec) {
   HB_CHECK(op == HB_OP_MTDAGMT || op == HB_OP_WDAGW, HB_E_ARG,
            "cg: operator must be HB_OP_MTDAGMT or HB_OP_WDAGW (got %d)", op);
   HB_CHECK(maxiter >= 0, HB_E_ARG, "cg: maxiter < 0");
-  int rc = 0;
-  double2 *t = spinor_scratch(c, SLOT_T, &rc); HB_TRY(rc);
-  double2 *y = spinor_scratch(c, SLOT_Y, &rc); HB_TRY(rc);
-  double2 *r = spinor_scratch(c, SLOT_R, &rc); HB_TRY(rc);
-  double2 *p = spinor_scratch(c, SLOT_P, &rc); HB_TRY(rc);
-  double2 *q = spinor_scratch(c, SLOT_Q, &rc); HB_TRY(rc);
-  CgState *st = static_cast<CgState *>(scratch(c, SLOT_STATE, sizeof(CgState), &rc)); HB_TRY(rc);
-  HB_TRY(ensure_host_stage(c, sizeof(CgState)));
-  CgState *hs = static_cast<CgState *>(c->host_stage);
-
-  EventTimer tm;
-  HB_TRY(tm.create());
-  HB_TRY(tm.start(c->stream));
-
-  CgState init{};
-  init.rest = (double)rest;
-  init.maxiter = maxiter;
-  *hs = init;
-  HB_HIP(hipMemcpyAsync(st, hs, sizeof(CgState), hipMemcpyHostToDevice, c->stream));
+  CgState *st = w.st, *hs = w.hs;
+  HB_TRY(cg_init(c, st, (double)rest, maxiter));
   // r = A x; r = b - r; p = r; rtr_old = |r|^2
-  HB_TRY(apply_f64(c, op, kappa, rho, r, x, t, y, nullptr, nullptr));
-  HB_TRY(cg_start(c, r, p, b, st));
+  HB_TRY(apply_f64(c, op, kappa, rho, w.r, x, w.t, w.y, nullptr, nullptr));
+  HB_TRY(cg_start(c, w.r, w.p, b, st, w.red));
   int *skip = &st->done;
   int nmv = 1;
   if (maxiter == 0) {
@@ -118,9 +105,9 @@ int cg_f64(Context *c, int op, double kappa, double rho, double2 *x, const doubl
     int chunk = 8;
     for (;;) {
       for (int k = 0; k < chunk; k++) {
-        HB_TRY(apply_f64(c, op, kappa, rho, q, p, t, y, &st->pap, skip));
-        HB_TRY(cg_update(c, x, r, p, q, st));
-        HB_TRY(cg_pupdate(c, p, r, st));
+        HB_TRY(apply_f64(c, op, kappa, rho, w.q, w.p, w.t, w.y, &st->pap, skip, w.red));
+        HB_TRY(cg_update(c, x, w.r, w.p, w.q, st, w.red));
+        HB_TRY(cg_pupdate(c, w.p, w.r, st));
       }
       HB_HIP(hipMemcpyAsync(hs, st, sizeof(CgState), hipMemcpyDeviceToHost, c->stream));
       HB_HIP(hipStreamSynchronize(c->stream));
@@ -129,24 +116,51 @@ int cg_f64(Context *c, int op, double kappa, double rho, double2 *x, const doubl
     }
     nmv += hs->iter;
   }
-  int niter = hs->iter;
+  const int niter = hs->iter;
+  const int done = hs->done;
+  const double rtr = hs->rtr;
   // true residual |b - A x|^2 in FP64 (invert_quda.F90:155-168)
-  HB_TRY(apply_f64(c, op, kappa, rho, q, x, t, y, nullptr, nullptr));
-  HB_TRY(residual(c, q, b, &st->res[0]));
-  HB_TRY(tm.stop(c->stream));
+  HB_TRY(apply_f64(c, op, kappa, rho, w.q, x, w.t, w.y, nullptr, nullptr));
+  HB_TRY(residual(c, w.q, b, &st->res[0], w.red));
   HB_HIP(hipMemcpyAsync(hs, st, sizeof(CgState), hipMemcpyDeviceToHost, c->stream));
   HB_HIP(hipStreamSynchronize(c->stream));
-  float ms = 0.f;
-  HB_TRY(tm.elapsed(&ms));
-  c->last_solve_ms = ms;
-  c->last_n_matvec = nmv + 1;
+  if (n_matvec) *n_matvec = nmv + 1;
   if (iters) *iters = niter;
   if (true_rtr) *true_rtr = hs->res[0];
-  if (maxiter > 0 && hs->done != 1 && log != 2) {
-    set_error("cg(): no convergence; rtr = %.17g", hs->rtr);
+  if (maxiter > 0 && done != 1 && log != 2) {
+    set_error("cg(): no convergence; rtr = %.17g", rtr);
     return HB_E_NOCONV;
   }
   return HB_OK;
+}
+
+// FP64 CG on device fields with the context's scratch slots, timed.
+int cg_f64(Context *c, int op, double kappa, double rho, double2 *x, const double2 *b, float rest,
+           int maxiter, int log, int *iters, double *true_rtr) {
+  int rc = 0;
+  CgWork w;
+  w.t = spinor_scratch(c, SLOT_T, &rc); HB_TRY(rc);
+  w.y = spinor_scratch(c, SLOT_Y, &rc); HB_TRY(rc);
+  w.r = spinor_scratch(c, SLOT_R, &rc); HB_TRY(rc);
+  w.p = spinor_scratch(c, SLOT_P, &rc); HB_TRY(rc);
+  w.q = spinor_scratch(c, SLOT_Q, &rc); HB_TRY(rc);
+  w.st = static_cast<CgState *>(scratch(c, SLOT_STATE, sizeof(CgState), &rc)); HB_TRY(rc);
+  w.red = c->red;
+  HB_TRY(ensure_host_stage(c, sizeof(CgState)));
+  w.hs = static_cast<CgState *>(c->host_stage);
+
+  EventTimer tm;
+  HB_TRY(tm.create());
+  HB_TRY(tm.start(c->stream));
+  int nmv = 0;
+  int r = cg_f64_run(c, w, op, kappa, rho, x, b, rest, maxiter, log, iters, true_rtr, &nmv);
+  if (r != HB_OK && r != HB_E_NOCONV) return r;
+  HB_TRY(tm.stop(c->stream));
+  float ms = 0.f;
+  HB_TRY(tm.elapsed(&ms));
+  c->last_solve_ms = ms;
+  c->last_n_matvec = nmv;
+  return r;
 }
 
 }  // namespace hb

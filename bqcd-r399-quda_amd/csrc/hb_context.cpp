@@ -105,6 +105,7 @@ int hb_init(const int L[4], const int bc[4], int device, hb_context **out) {
     return HB_E_HIP;
   }
   c->stream = c->own_stream;
+  noalloc_setup(device);
   // reduction partials: up to 16 values per block of a site-parallel grid
   c->red_blocks = nblocks(2 * c->g.vh) + 64;
   e = hipMalloc(&c->red, sizeof(double) * 16 * (size_t)c->red_blocks);

@@ -167,6 +167,8 @@ struct Context {
 int ensure_dev_stage(Context *c, size_t bytes);
 int ensure_host_stage(Context *c, size_t bytes);
 void *scratch(Context *c, int slot, size_t bytes, int *rc);
+// looks up the device address of the no-alloc CG's static work area (hb_noalloc.hip)
+void noalloc_setup(int device);
 
 constexpr int kBlock = 256;
 

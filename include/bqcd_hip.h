@@ -180,6 +180,19 @@ int hb_apply_d(hb_context *ctx, int op, double kappa, double rho, double *out_d,
                const double *in_d);
 int hb_cg_d(hb_context *ctx, int op, double kappa, double rho, double *x_d, const double *b_d,
             float rest, int maxiter, int log, int *iters, double *true_rtr);
+/* hb_cg_d without any allocation.  It has no counterpart in the reference; it
+ * is for callers that cannot allocate device memory (code under stream
+ * capture, tight solver loops, harnesses that forbid hipMalloc).  Arguments,
+ * kernels and results are those of hb_cg_d.  Its work fields (five spinors,
+ * the CG state, the reduction partials) live in one static device array per
+ * GPU, sized for volh <= HB_NOALLOC_MAX_VOLH (60 MiB, held until the process
+ * ends); a larger lattice returns HB_E_ARG.  Every call on one GPU uses that
+ * GPU's array: calls that overlap on one GPU race.  hb_last_timing reports
+ * its operator count, not its time. */
+#define HB_NOALLOC_MAX_VOLH 65536
+int hb_cg_d_noalloc(hb_context *ctx, int op, double kappa, double rho, double *x_d,
+                    const double *b_d, float rest, int maxiter, int log, int *iters,
+                    double *true_rtr);
 int hb_cg_mixed_d(hb_context *ctx, int op, double kappa, double rho, double *x_d,
                   const double *b_d, float rest, int maxiter, int log, int precision,
                   double delta, int max_inner, int *iters, int *n_reliable, double *true_rtr);
