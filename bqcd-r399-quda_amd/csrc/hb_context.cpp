@@ -130,7 +130,6 @@ int hb_finalize(hb_context *ctx) {
   for (auto *p : c->clov32) (void)hipFree(p);
   (void)hipFree(c->gauge16);
   for (auto *p : c->clov16) (void)hipFree(p);
-  for (auto *p : c->clov16n) (void)hipFree(p);
   for (auto &f : c->scratch) (void)hipFree(f.p);
   (void)hipFree(c->red);
   (void)hipFree(c->dev_stage);

@@ -37,10 +37,7 @@ __global__ __launch_bounds__(kBlock) void k_dslash(DslashArgs<S, G> a) {
     const SpinorView<S> in(a.in, vh);
     if constexpr (HB_WAVE_SHARE && std::is_same<S, SF16>::value) {
       uint32_t w[13];
-      const uint32_t vo = (uint32_t)i * 4u;
-#pragma unroll
-      for (int k = 0; k < 12; k++) w[k] = __builtin_amdgcn_raw_buffer_load_b32(in.r, vo, in.plane * k, 0);
-      w[12] = __builtin_amdgcn_raw_buffer_load_b32(in.rn, vo, 0, 0);
+      in.load_words(w, i);
       const WaveView16 wv{in, w, i - (int)(threadIdx.x & 63), g.L[0] / 2 <= 32};
       hop_all<S, G, DAG, 1, WaveView16>(acc, a.U, wv, g, e, i, xc);
     } else {

@@ -136,8 +136,7 @@ struct Context {
   float2 *gauge32 = nullptr;
   float2 *clov32[2] = {nullptr, nullptr};          // A, A^-1
   short2 *gauge16 = nullptr;                       // 16-bit fixed point
-  short2 *clov16[2] = {nullptr, nullptr};
-  float *clov16n[2] = {nullptr, nullptr};          // per site and block: scale, diagonal offset
+  short2 *clov16[2] = {nullptr, nullptr};          // words and (scale, diagonal offset) per block
   uint64_t gauge_version = 1, clover_version = 1;
   uint64_t lp_gauge_version = 0, lp_clover_version = 0;
   // scratch

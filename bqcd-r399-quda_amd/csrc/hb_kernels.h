@@ -19,6 +19,12 @@ using rsrc_t = __amdgpu_buffer_rsrc_t;
 typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
 typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
 
+// a stored 32-bit word as a short2 / float.  Taken by value: hipcc reads only
+// element 0 of a vector when __builtin_bit_cast is applied to an element of
+// it directly (q[k] of a u32x4), so every word goes through a scalar first.
+__device__ __forceinline__ short2 as_short2(uint32_t w) { return __builtin_bit_cast(short2, w); }
+__device__ __forceinline__ float as_float(uint32_t w) { return __builtin_bit_cast(float, w); }
+
 __device__ __forceinline__ rsrc_t make_rsrc(const void *p, uint32_t bytes) {
   return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(p), (short)0, (int)bytes, 0x00020000);
 }
@@ -181,6 +187,86 @@ template <class S> struct SpinorView {
   }
 };
 
+// 16-bit fields are stored as planes of 16-byte vectors: plane q holds 16
+// bytes (four short2 words) per element, so a 16-byte load of a wave reads
+// 1 KiB of contiguous memory -- the cache lines that one 4-byte plane per word
+// would touch, in a quarter of the load instructions.
+//   spinor (n elements: sites, or (site, RHS) pairs of a block field): three
+//     planes, plane q = components 4q .. 4q+3 (component k = 3*spin + colour);
+//     the FP32 element scales in a separate array.  A full spinor is three
+//     16-byte loads, a half spinor (two spins, the t projector) a 16- and an
+//     8-byte load.
+//   link slab (mu, eo): a 16-byte plane (words 0-3 of rows 0 and 1) and an
+//     8-byte plane (words 4, 5)
+//   clover slab (eo, J): five 16-byte planes, words 4q .. 4q+3; the last holds
+//     words 16, 17 and the block's factors (m, c)
+template <int POL = 0>
+__device__ __forceinline__ u32x4 bld_u32x4(rsrc_t r, uint32_t vo, uint32_t so) {
+  return __builtin_bit_cast(u32x4, __builtin_amdgcn_raw_buffer_load_b128(r, vo, so, POL));
+}
+template <int POL = 0>
+__device__ __forceinline__ u32x2 bld_u32x2(rsrc_t r, uint32_t vo, uint32_t so) {
+  return __builtin_bit_cast(u32x2, __builtin_amdgcn_raw_buffer_load_b64(r, vo, so, POL));
+}
+
+template <> struct SpinorView<SF16> {
+  rsrc_t r, rn;
+  uint32_t plane;  // bytes per 16-byte plane
+  __device__ __forceinline__ SpinorView(const SpinorPtr<SF16> &f, int n) {
+    plane = (uint32_t)n * 16u;
+    r = make_rsrc(f.p, plane * 3u);
+    rn = make_rsrc(f.norm, (uint32_t)n * 4u);
+  }
+  // the 12 stored words of element j and its scale word
+  __device__ __forceinline__ void load_words(uint32_t (&w)[13], int j) const {
+    const uint32_t vo = (uint32_t)j * 16u;
+    const u32x4 a = bld_u32x4(r, vo, 0), b = bld_u32x4(r, vo, plane), c = bld_u32x4(r, vo, 2 * plane);
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+      w[k] = a[k];
+      w[4 + k] = b[k];
+      w[8 + k] = c[k];
+    }
+    w[12] = __builtin_amdgcn_raw_buffer_load_b32(rn, (uint32_t)j * 4u, 0, 0);
+  }
+  // the stored integers as floats and the factor (scale / 32767) still to be
+  // applied (the hopping term applies it once to the projected half spinor)
+  __device__ __forceinline__ float load_raw(float2 (&v)[12], int j) const {
+    uint32_t w[13];
+    load_words(w, j);
+#pragma unroll
+    for (int k = 0; k < 12; k++) v[k] = SF16::raw(as_short2(w[k]));
+    return as_float(w[12]) * SF16::kInv;
+  }
+  // spin components s0, s0+1 (k = 3*s0 .. 3*s0+5): words 0-5 (plane 0 and the
+  // first half of plane 1) or 6-11 (the second half of plane 1 and plane 2)
+  __device__ __forceinline__ float load_pair_raw(float2 (&v)[6], int j, int s0) const {
+    const uint32_t vo = (uint32_t)j * 16u;
+    uint32_t w[6];
+    if (s0 == 0) {
+      const u32x4 a = bld_u32x4(r, vo, 0);
+      const u32x2 b = bld_u32x2(r, vo, plane);
+      w[0] = a[0]; w[1] = a[1]; w[2] = a[2]; w[3] = a[3]; w[4] = b[0]; w[5] = b[1];
+    } else {
+      const u32x2 a = bld_u32x2(r, vo + 8u, plane);
+      const u32x4 b = bld_u32x4(r, vo, 2 * plane);
+      w[0] = a[0]; w[1] = a[1]; w[2] = b[0]; w[3] = b[1]; w[4] = b[2]; w[5] = b[3];
+    }
+#pragma unroll
+    for (int k = 0; k < 6; k++) v[k] = SF16::raw(as_short2(w[k]));
+    return bld_f32(rn, (uint32_t)j * 4u, 0) * SF16::kInv;
+  }
+  __device__ __forceinline__ void load(float2 (&v)[12], int j) const {
+    const float f = load_raw(v, j);
+#pragma unroll
+    for (int k = 0; k < 12; k++) { v[k].x *= f; v[k].y *= f; }
+  }
+  template <int MU>
+  __device__ __forceinline__ float load_raw_dir(float2 (&v)[12], int j) const {
+    return load_raw(v, j);
+  }
+};
+
 // 16-bit neighbour spinors shared through the wavefront: every lane holds the
 // stored words of the input site with its own index (13 words), and a hop in x
 // or y whose neighbour index lies in the same wave takes the words from that
@@ -212,8 +298,8 @@ struct WaveView16 {
       float f;
       if (inwave) {
 #pragma unroll
-        for (int k = 0; k < 12; k++) v[k] = SF16::raw(__builtin_bit_cast(short2, x[k]));
-        f = __builtin_bit_cast(float, x[12]) * SF16::kInv;
+        for (int k = 0; k < 12; k++) v[k] = SF16::raw(as_short2(x[k]));
+        f = as_float(x[12]) * SF16::kInv;
       } else {
         f = g.load_raw(v, j);
       }
@@ -226,10 +312,14 @@ template <class S>
 __device__ __forceinline__ void load_spinor(typename S::V (&v)[12], const SpinorPtr<S> &f, int i,
                                             int vh) {
   if constexpr (S::kNorm) {
+    // three 16-byte planes (see SpinorView<SF16>)
     const float n = f.norm[i] * S::kInv;
+    const u32x4 *P = reinterpret_cast<const u32x4 *>(f.p);
+    const u32x4 q[3] = {P[i], P[(size_t)vh + i], P[2 * (size_t)vh + i]};
 #pragma unroll
     for (int k = 0; k < 12; k++) {
-      v[k] = S::raw(f.p[(size_t)k * vh + i]);
+      const uint32_t w = q[k >> 2][k & 3];
+      v[k] = S::raw(as_short2(w));
       v[k].x *= n;
       v[k].y *= n;
     }
@@ -257,16 +347,22 @@ __device__ __forceinline__ void store_spinor(const SpinorPtr<S> &f, int i, int v
 #pragma unroll
     for (int k = 0; k < 12; k++) m = fmaxf(m, fmaxf(fabsf((float)v[k].x), fabsf((float)v[k].y)));
     float inv = m > 0.f ? 1.0f / m : 0.f;
+    // three 16-byte planes (see SpinorView<SF16>): three 16-byte stores
+    uint32_t w[12];
+#pragma unroll
+    for (int k = 0; k < 12; k++)
+      w[k] = __builtin_bit_cast(uint32_t, S::qpack((float)v[k].x * inv, (float)v[k].y * inv));
+    u32x4 *P = reinterpret_cast<u32x4 *>(f.p);
 #if HB_NT_STORE_REDUCED
     __builtin_nontemporal_store(m, &f.norm[i]);
 #pragma unroll
-    for (int k = 0; k < 12; k++)
-      __builtin_nontemporal_store(__builtin_bit_cast(uint32_t, S::qpack((float)v[k].x * inv, (float)v[k].y * inv)),
-                                  reinterpret_cast<uint32_t *>(f.p + (size_t)k * vh + i));
+    for (int q = 0; q < 3; q++)
+      __builtin_nontemporal_store(u32x4{w[4 * q], w[4 * q + 1], w[4 * q + 2], w[4 * q + 3]},
+                                  &P[(size_t)q * vh + i]);
 #else
     f.norm[i] = m;
 #pragma unroll
-    for (int k = 0; k < 12; k++) f.p[(size_t)k * vh + i] = S::qpack((float)v[k].x * inv, (float)v[k].y * inv);
+    for (int q = 0; q < 3; q++) P[(size_t)q * vh + i] = u32x4{w[4 * q], w[4 * q + 1], w[4 * q + 2], w[4 * q + 3]};
 #endif
   } else if constexpr (sizeof(typename S::ST) == 16) {
 #pragma unroll
@@ -283,11 +379,15 @@ __device__ __forceinline__ void store_spinor(const SpinorPtr<S> &f, int i, int v
 // U in SU(3) the third row is conj(row0 x row1), rebuilt after the load, so a
 // link costs 6 instead of 9 complex numbers of traffic.
 // ----------------------------------------------------------------------------
+//
+// 16-bit links are stored in a 16-byte and an 8-byte plane per (mu, eo) slab
+// instead (24 bytes per site, see SpinorView<SF16>), one 16- and one 8-byte load.
 template <class S> constexpr int link_planes() { return S::kRecon ? 6 : 9; }
 
 template <class S, int POL = kNT>
 __device__ __forceinline__ void load_link(typename S::V (&u)[9], const typename S::ST *U, int mu,
                                           int eo, int i, int vh) {
+  static_assert(!std::is_same<S, SF16>::value, "16-bit links are read with load_link_f");
   constexpr int NP = link_planes<S>();
   const uint32_t plane = (uint32_t)vh * S::kBytes;
   rsrc_t r = make_rsrc(U + (size_t)((mu * 2 + eo) * NP) * vh, plane * (uint32_t)NP);
@@ -318,12 +418,12 @@ __device__ __forceinline__ typename G::A load_link_f(typename G::V (&u)[9], cons
                                                      int mu, int eo, int i, int vh) {
   if constexpr (link_raw<G>()) {
     using V = typename G::V;
-    constexpr int NP = link_planes<G>();
-    const uint32_t plane = (uint32_t)vh * G::kBytes;
-    rsrc_t r = make_rsrc(U + (size_t)((mu * 2 + eo) * NP) * vh, plane * (uint32_t)NP);
-    const uint32_t vo = (uint32_t)i * G::kBytes;
+    rsrc_t r = make_rsrc(U + (size_t)((mu * 2 + eo) * 6) * vh, (uint32_t)vh * 24u);
+    const u32x4 a = bld_u32x4<POL>(r, (uint32_t)i * 16u, 0);
+    const u32x2 b = bld_u32x2<POL>(r, (uint32_t)i * 8u, (uint32_t)vh * 16u);
+    const uint32_t w[6] = {a[0], a[1], a[2], a[3], b[0], b[1]};
 #pragma unroll
-    for (int k = 0; k < NP; k++) u[k] = G::template bld_raw<POL>(r, vo, plane * k);
+    for (int k = 0; k < 6; k++) u[k] = G::raw(as_short2(w[k]));
     auto cross = [](V a0, V b1, V a1, V b0) {  // conj(a0 b1 - a1 b0) / 32767
       V t = cmul(a0, b1);
       V s = cmul(a1, b0);
@@ -518,41 +618,88 @@ __device__ __forceinline__ float2 clover_block_word(double2 v, int k, float2 mc)
   return make_float2((float)v.x * inv, (float)v.y * inv);
 }
 
-// loads block J of the clover field of parity eo at site i
+// loads block J of the FP64 / FP32 clover field of parity eo at site i (16-bit
+// fields are site-packed, see load_clover_block_raw)
 template <class S>
 __device__ __forceinline__ void load_clover_block(typename S::V (&w)[18], const typename S::ST *C,
                                                   const float *Cn, int eo, int J, int i, int vh) {
+  static_assert(!S::kNorm, "16-bit clover blocks are read with load_clover_block_raw");
   const uint32_t plane = (uint32_t)vh * S::kBytes;
   rsrc_t r = make_rsrc(C + (size_t)(eo * 36 + J * 18) * vh, plane * 18u);
   const uint32_t vo = (uint32_t)i * S::kBytes;
 #pragma unroll
   for (int k = 0; k < 18; k++) w[k] = S::template bld<kNT>(r, vo, plane * k);
-  if constexpr (S::kNorm) {
-    const float2 mc = reinterpret_cast<const float2 *>(Cn)[(size_t)(eo * 2 + J) * vh + i];
-#pragma unroll
-    for (int k = 0; k < 18; k++) {
-      w[k].x *= mc.x;
-      w[k].y *= mc.x;
-      if (clover_diag_word(k)) {
-        w[k].x += mc.y;
-        w[k].y += mc.y;
-      }
-    }
-  }
 }
 
-// block J as stored in 16 bits (the integers Q as floats) and its factors:
-// the block is c 1 + (m / 32767) Q_int; returns (m / 32767, c)
+// 16-bit clover blocks: per block and element five 16-byte planes (see
+// SpinorView<SF16>): the 18 words of Q as short2, then (m, c) as a float2.
+// Field layout [eo][J] slabs of vh elements; the multi-RHS solver's slabs hold
+// its kappa groups, element gidx * vh + i.
+constexpr uint32_t kClov16 = 80;  // bytes of one block of one element
+constexpr int kClov16Words = 20;  // the same in short2 units
+
+// block of element e of a slab (buffer r, plane = 16 * elements per slab
+// bytes): Q as stored (the integers as floats) and the block's factors: the
+// block is c 1 + (m / 32767) Q_int; returns (m / 32767, c)
+__device__ __forceinline__ float2 load_clover_rec16(float2 (&w)[18], rsrc_t r, uint32_t e, uint32_t plane) {
+  u32x4 q[5];
+#pragma unroll
+  for (int k = 0; k < 5; k++) q[k] = bld_u32x4<HB_REDUCED_OPERATOR_POLICY>(r, e * 16u, plane * k);
+#pragma unroll
+  for (int k = 0; k < 18; k++) {
+    const uint32_t x = q[k >> 2][k & 3];
+    w[k] = SF16::raw(as_short2(x));
+  }
+  const uint32_t m = q[4][2], c = q[4][3];
+  return make_float2(as_float(m) * SF16::kInv, as_float(c));
+}
+
+// block J of the 16-bit clover field of parity eo at site i (Cn is unused: the
+// factors are stored with the words)
 __device__ __forceinline__ float2 load_clover_block_raw(float2 (&w)[18], const SF16::ST *C,
                                                         const float *Cn, int eo, int J, int i,
                                                         int vh) {
-  const uint32_t plane = (uint32_t)vh * SF16::kBytes;
-  rsrc_t r = make_rsrc(C + (size_t)(eo * 36 + J * 18) * vh, plane * 18u);
-  const uint32_t vo = (uint32_t)i * SF16::kBytes;
+  rsrc_t r = make_rsrc(C + (size_t)((eo * 2 + J) * kClov16Words) * vh, (uint32_t)vh * kClov16);
+  return load_clover_rec16(w, r, (uint32_t)i, (uint32_t)vh * 16u);
+}
+
+// writes the block of element e of a slab of n elements: the words Q (in
+// [-1, 1]) and the factors (m, c)
+__device__ __forceinline__ void store_clover_rec16(short2 *slab, size_t e, size_t n, const float2 (&q)[18],
+                                                   float2 mc) {
+  uint32_t w[20];
 #pragma unroll
-  for (int k = 0; k < 18; k++) w[k] = SF16::bld_raw<HB_REDUCED_OPERATOR_POLICY>(r, vo, plane * k);
-  const float2 mc = reinterpret_cast<const float2 *>(Cn)[(size_t)(eo * 2 + J) * vh + i];
-  return make_float2(mc.x * SF16::kInv, mc.y);
+  for (int k = 0; k < 18; k++) w[k] = __builtin_bit_cast(uint32_t, make_short2(SF16::q(q[k].x), SF16::q(q[k].y)));
+  w[18] = __builtin_bit_cast(uint32_t, mc.x);
+  w[19] = __builtin_bit_cast(uint32_t, mc.y);
+  u32x4 *P = reinterpret_cast<u32x4 *>(slab);
+#pragma unroll
+  for (int k = 0; k < 5; k++) P[(size_t)k * n + e] = u32x4{w[4 * k], w[4 * k + 1], w[4 * k + 2], w[4 * k + 3]};
+}
+
+// 16-bit form of the FP64 block s (words s[k * stride]) as element e of a slab
+// of n elements, see clover_block_scale
+__device__ __forceinline__ void clover_to_rec16(short2 *slab, size_t e, size_t n, const double2 *s,
+                                                size_t stride) {
+  const float2 mc = clover_block_scale(s, stride);
+  float2 q[18];
+#pragma unroll
+  for (int k = 0; k < 18; k++) q[k] = clover_block_word(s[(size_t)k * stride], k, mc);
+  store_clover_rec16(slab, e, n, q, mc);
+}
+
+// links in 16 bits: rows 0 and 1 of U (FP64 words s[k * stride]) as site i of
+// a (mu, eo) slab of vh sites (a 16-byte and an 8-byte plane, see load_link_f)
+__device__ __forceinline__ void link_to_rec16(short2 *slab, size_t i, size_t vh, const double2 *s,
+                                              size_t stride) {
+  uint32_t w[6];
+#pragma unroll
+  for (int k = 0; k < 6; k++) {
+    const double2 v = s[(size_t)k * stride];
+    w[k] = __builtin_bit_cast(uint32_t, make_short2(SF16::q((float)v.x), SF16::q((float)v.y)));
+  }
+  reinterpret_cast<u32x4 *>(slab)[i] = u32x4{w[0], w[1], w[2], w[3]};
+  reinterpret_cast<u32x2 *>(slab + 4 * vh)[i] = u32x2{w[4], w[5]};
 }
 
 // y = (c 1 + f Q) x for one block from y = Q x (mc = (f, c))

@@ -46,17 +46,22 @@ __global__ void k_cvt_gauge_rows(typename S::ST *__restrict__ dst, const double2
   }
 }
 
+// links in 16 bits: rows 0 and 1, a 16-byte and an 8-byte plane per (mu, eo)
+// slab (see load_link_f)
+__global__ void k_cvt_gauge16(short2 *__restrict__ dst, const double2 *__restrict__ src, int vh) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int slab = blockIdx.y;  // mu*2 + eo
+  if (i >= vh) return;
+  link_to_rec16(dst + (size_t)slab * 6 * vh, i, vh, src + (size_t)slab * 9 * vh + i, vh);
+}
+
 // clover in 16-bit storage: per site and block, c 1 + m Q (see clover_block_scale)
-__global__ void k_cvt_clover16(short2 *__restrict__ dst, float2 *__restrict__ nrm,
-                               const double2 *__restrict__ src, int vh) {
+// in five 16-byte planes per [eo][J] slab (see load_clover_block_raw)
+__global__ void k_cvt_clover16(short2 *__restrict__ dst, const double2 *__restrict__ src, int vh) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   int eb = blockIdx.y;  // eo*2 + J
   if (i >= vh) return;
-  const double2 *s = src + (size_t)eb * 18 * vh + i;
-  const float2 mc = clover_block_scale(s, vh);
-  nrm[(size_t)eb * vh + i] = mc;
-  for (int k = 0; k < 18; k++)
-    SF16::st(dst, (size_t)eb * 18 * vh + (size_t)k * vh + i, clover_block_word(s[(size_t)k * vh], k, mc));
+  clover_to_rec16(dst + (size_t)eb * kClov16Words * vh, i, vh, src + (size_t)eb * 18 * vh + i, vh);
 }
 
 // spinor FP64 -> S (thread per site: FP16 needs the site's norm)
@@ -235,7 +240,7 @@ template <> struct LP<SF32> {
 template <> struct LP<SF16> {
   static const short2 *gauge(Context *c) { return c->gauge16; }
   static const short2 *clov(Context *c, int k) { return c->clov16[k]; }
-  static const float *norm(Context *c, int k) { return c->clov16n[k]; }
+  static const float *norm(Context *, int) { return nullptr; }  // factors stored with the words
 };
 
 int ensure_lowprec(Context *c, int prec) {
@@ -258,15 +263,12 @@ int ensure_lowprec(Context *c, int prec) {
     }
   } else {
     if (!c->gauge16) HB_HIP(hipMalloc(&c->gauge16, sizeof(short2) * 48 * vh));
-    k_cvt_gauge_rows<SF16><<<grid, kBlock, 0, c->stream>>>(c->gauge16, c->gauge, vh);
+    k_cvt_gauge16<<<dim3(nblocks((int)vh), 8), kBlock, 0, c->stream>>>(c->gauge16, c->gauge, (int)vh);
     HB_HIP(hipGetLastError());
     if (clov) {
       for (int k = 0; k < 2; k++) {
-        if (!c->clov16[k]) HB_HIP(hipMalloc(&c->clov16[k], sizeof(short2) * 72 * vh));
-        if (!c->clov16n[k]) HB_HIP(hipMalloc(&c->clov16n[k], sizeof(float2) * 4 * vh));
-        k_cvt_clover16<<<dim3(nblocks((int)vh), 4), kBlock, 0, c->stream>>>(c->clov16[k],
-                                                                          reinterpret_cast<float2 *>(c->clov16n[k]),
-                                                                          c->clov[k], (int)vh);
+        if (!c->clov16[k]) HB_HIP(hipMalloc(&c->clov16[k], sizeof(short2) * 4 * kClov16Words * vh));
+        k_cvt_clover16<<<dim3(nblocks((int)vh), 4), kBlock, 0, c->stream>>>(c->clov16[k], c->clov[k], (int)vh);
         HB_HIP(hipGetLastError());
       }
     }

@@ -38,8 +38,9 @@ template <class S, class G> struct MultiArgs {
   int out_parity;
   SpinorPtr<S> out, in, x0, dotv;   // block layout, NR RHS per site
   const typename G::ST *U;
-  const typename G::ST *C;          // [ngroups][2][36][vh] (A^-1 for CLINV, A for MTIL)
-  const float *Cn;                  // [ngroups][2][2][vh] x (scale, diagonal offset)
+  const typename G::ST *C;          // A^-1 for CLINV, A for MTIL: FP32 planes [ngroups][2][36][vh],
+                                    // 16-bit [2][2] slabs of ngroups * vh elements (load_clover_block_g_raw)
+  const float *Cn;                  // unused (16-bit records carry their factors)
   int ngroups;
   int grp[16];
   double mk2[16], rho[16];
@@ -48,44 +49,30 @@ template <class S, class G> struct MultiArgs {
   double *partials;
 };
 
-// clover block J of parity eo at site i for kappa group gidx
+// clover block J of parity eo at site i for kappa group gidx (FP32 planes)
 template <class SG>
 __device__ __forceinline__ void load_clover_block_g(typename SG::V (&w)[18], const typename SG::ST *C,
                                                     const float *Cn, int ngroups, int gidx, int eo,
                                                     int J, int i, int vh) {
+  static_assert(!SG::kNorm, "16-bit clover blocks are read with load_clover_block_g_raw");
   const uint32_t plane = (uint32_t)vh * SG::kBytes;
   rsrc_t r = make_rsrc(C, plane * 72u * (uint32_t)ngroups);
   const uint32_t vo = ((uint32_t)gidx * 72u * (uint32_t)vh + (uint32_t)i) * SG::kBytes;
   const uint32_t base = (uint32_t)(eo * 36 + J * 18);
 #pragma unroll
   for (int k = 0; k < 18; k++) w[k] = SG::template bld<kNT>(r, vo, plane * (base + k));
-  if constexpr (SG::kNorm) {
-    const float2 mc = reinterpret_cast<const float2 *>(Cn)[((size_t)gidx * 4 + eo * 2 + J) * vh + i];
-#pragma unroll
-    for (int k = 0; k < 18; k++) {
-      w[k].x *= mc.x;
-      w[k].y *= mc.x;
-      if (clover_diag_word(k)) {
-        w[k].x += mc.y;
-        w[k].y += mc.y;
-      }
-    }
-  }
 }
 
-// the same block as stored in 16 bits (integers as floats); returns
-// (m / 32767, c) -- the block is c 1 + (m / 32767) Q_int
+// the same block of a 16-bit field: [eo][J] slabs of 16-byte planes whose
+// elements are gidx * vh + i (one buffer per slab spans all groups; see
+// load_clover_rec16); returns (m / 32767, c) -- the block is
+// c 1 + (m / 32767) Q_int.  Cn is unused.
 __device__ __forceinline__ float2 load_clover_block_g_raw(float2 (&w)[18], const SF16::ST *C,
                                                           const float *Cn, int ngroups, int gidx,
                                                           int eo, int J, int i, int vh) {
-  const uint32_t plane = (uint32_t)vh * SF16::kBytes;
-  rsrc_t r = make_rsrc(C, plane * 72u * (uint32_t)ngroups);
-  const uint32_t vo = ((uint32_t)gidx * 72u * (uint32_t)vh + (uint32_t)i) * SF16::kBytes;
-  const uint32_t base = (uint32_t)(eo * 36 + J * 18);
-#pragma unroll
-  for (int k = 0; k < 18; k++) w[k] = SF16::bld_raw<kNT>(r, vo, plane * (base + k));
-  const float2 mc = reinterpret_cast<const float2 *>(Cn)[((size_t)gidx * 4 + eo * 2 + J) * vh + i];
-  return make_float2(mc.x * SF16::kInv, mc.y);
+  const size_t slab = (size_t)ngroups * vh;
+  rsrc_t r = make_rsrc(C + (size_t)(eo * 2 + J) * slab * kClov16Words, (uint32_t)(slab * kClov16));
+  return load_clover_rec16(w, r, (uint32_t)gidx * (uint32_t)vh + (uint32_t)i, (uint32_t)slab * 16u);
 }
 
 template <class SG, class V>
@@ -153,10 +140,7 @@ __global__ __launch_bounds__(kBlock) void k_dslash_multi(MultiArgs<S, G> a) {
       // From NR = 4 on a wave spans 16 sites or fewer, and the extra registers of
       // the own-index spinor cost more than the shared x hops save (measured)
       uint32_t w[13];
-      const uint32_t vo = (uint32_t)tid * 4u;
-#pragma unroll
-      for (int k = 0; k < 12; k++) w[k] = __builtin_amdgcn_raw_buffer_load_b32(in.r, vo, in.plane * k, 0);
-      w[12] = __builtin_amdgcn_raw_buffer_load_b32(in.rn, vo, 0, 0);
+      in.load_words(w, tid);
       const WaveView16 wv{in, w, tid - (int)(threadIdx.x & 63), g.L[0] / 2 * NR <= 32};
       hop_all<S, G, DAG, NR, WaveView16>(acc, a.U, wv, g, e, site, xc, r);
     } else {
@@ -382,34 +366,42 @@ __global__ __launch_bounds__(kBlock) void k_multi_prestart(SpinorPtr<S> p, Spino
 }
 
 // conversions of the per-group clover fields --------------------------------------
+// group gi of ngroups: FP32 planes [group][2][36][vh]; 16-bit [eo][J] slabs of
+// 16-byte planes, element gi * vh + i (see load_clover_block_g_raw)
 template <class S>
-__global__ void k_cvt_clover_lp(typename S::ST *__restrict__ dst, float *__restrict__ nrm,
-                                const double2 *__restrict__ src, int vh) {
+__global__ void k_cvt_clover_lp(typename S::ST *__restrict__ dst, const double2 *__restrict__ src, int vh,
+                                int ngroups, int gi) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   const int eb = blockIdx.y;  // eo*2 + J
   if (i >= vh) return;
   const double2 *s = src + (size_t)eb * 18 * vh + i;
   if constexpr (S::kNorm) {
-    const float2 mc = clover_block_scale(s, vh);
-    reinterpret_cast<float2 *>(nrm)[(size_t)eb * vh + i] = mc;
-    for (int k = 0; k < 18; k++)
-      S::st(dst, (size_t)eb * 18 * vh + (size_t)k * vh + i, clover_block_word(s[(size_t)k * vh], k, mc));
+    const size_t n = (size_t)ngroups * vh;
+    clover_to_rec16(dst + (size_t)eb * kClov16Words * n, (size_t)gi * vh + i, n, s, vh);
   } else {
+    typename S::ST *d = dst + (size_t)gi * 72 * vh;
     for (int k = 0; k < 18; k++) {
       double2 v = s[(size_t)k * vh];
-      S::st(dst, (size_t)eb * 18 * vh + (size_t)k * vh + i, make_float2((float)v.x, (float)v.y));
+      S::st(d, (size_t)eb * 18 * vh + (size_t)k * vh + i, make_float2((float)v.x, (float)v.y));
     }
   }
 }
 
-// links in reduced precision, rows 0 and 1 only (U[4][2][6][vh], see load_link)
+// links in reduced precision, rows 0 and 1 only: FP32 planes U[4][2][6][vh]
+// (see load_link), 16-bit a 16-byte and an 8-byte plane per (mu, eo) slab
 template <class S>
-__global__ void k_cvt_gauge_lp(typename S::ST *__restrict__ dst, const double2 *__restrict__ src, size_t vh) {
-  const size_t n = 48 * vh;
-  for (size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += (size_t)gridDim.x * blockDim.x) {
-    const size_t slab = k / (6 * vh), rem = k - slab * 6 * vh;
-    const double2 v = src[slab * 9 * vh + rem];
-    S::st(dst, k, make_float2((float)v.x, (float)v.y));
+__global__ void k_cvt_gauge_lp(typename S::ST *__restrict__ dst, const double2 *__restrict__ src, int vh) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int slab = blockIdx.y;  // mu*2 + eo
+  if (i >= vh) return;
+  const double2 *s = src + (size_t)slab * 9 * vh + i;
+  if constexpr (S::kNorm) {
+    link_to_rec16(dst + (size_t)slab * 6 * vh, i, vh, s, vh);
+  } else {
+    for (int k = 0; k < 6; k++) {
+      const double2 v = s[(size_t)k * vh];
+      S::st(dst, ((size_t)slab * 6 + k) * vh + i, make_float2((float)v.x, (float)v.y));
+    }
   }
 }
 
@@ -524,23 +516,26 @@ int multi_fields(Context *c, const MultiPlan &plan, MultiFields<S, G> &f) {
   static_assert(G::kRecon, "reduced-precision links are stored as two rows");
   typename G::ST *U = static_cast<typename G::ST *>(scratch(c, SLOT_MULTI + 20, st_bytes<G>() * 48 * vh, &rc));
   HB_TRY(rc);
-  k_cvt_gauge_lp<G><<<nblocks((int)std::min<size_t>(48 * vh, (size_t)1 << 22)), kBlock, 0, c->stream>>>(U, c->gauge, vh);
+  k_cvt_gauge_lp<G><<<dim3(nblocks((int)vh), 8), kBlock, 0, c->stream>>>(U, c->gauge, (int)vh);
   HB_HIP(hipGetLastError());
   f.U = U;
   if (!plan.clover) return HB_OK;
-  // the block kernels address the clover words of all kappa groups through one buffer
-  HB_CHECK(fits_buffer(72ULL * vh * plan.ngroups * st_bytes<G>()), HB_E_ARG,
+  // bytes of one buffer: FP32 planes of all groups; 16-bit one (eo, J) slab of
+  // all groups
+  const unsigned long long span = G::kNorm ? (unsigned long long)kClov16 * vh * plan.ngroups
+                                           : 72ULL * vh * plan.ngroups * st_bytes<G>();
+  HB_CHECK(fits_buffer(span), HB_E_ARG,
            "cg_multi: the clover terms of %d kappa values at %zu sites per parity exceed the 4 GiB "
            "span of one buffer", plan.ngroups, vh);
+  const size_t field = G::kNorm ? (size_t)4 * kClov16 * vh * plan.ngroups : st_bytes<G>() * 72 * vh * plan.ngroups;
   for (int w = 0; w < 2; w++) {
-    f.C[w] = static_cast<typename G::ST *>(scratch(c, SLOT_MULTI + 21 + w, st_bytes<G>() * 72 * vh * plan.ngroups, &rc));
+    f.C[w] = static_cast<typename G::ST *>(scratch(c, SLOT_MULTI + 21 + w, field, &rc));
     HB_TRY(rc);
-    f.Cn[w] = static_cast<float *>(scratch(c, SLOT_MULTI + 23 + w, sizeof(float2) * 4 * vh * plan.ngroups, &rc));
-    HB_TRY(rc);
+    f.Cn[w] = nullptr;
     for (int gi = 0; gi < plan.ngroups; gi++) {
       const double2 *src = w == 0 ? plan.clovA[gi] : plan.clovI[gi];
-      k_cvt_clover_lp<G><<<dim3(nblocks((int)vh), 4), kBlock, 0, c->stream>>>(
-          f.C[w] + (size_t)gi * 72 * vh, f.Cn[w] + (size_t)gi * 8 * vh, src, (int)vh);
+      k_cvt_clover_lp<G><<<dim3(nblocks((int)vh), 4), kBlock, 0, c->stream>>>(f.C[w], src, (int)vh,
+                                                                           plan.ngroups, gi);
       HB_HIP(hipGetLastError());
     }
   }
@@ -699,10 +694,35 @@ int cg_multi_block(Context *c, const MultiPlan &plan, const MultiFields<S, G> &f
 // workgroup size of the block kernels
 struct TuneKey {
   int L0, L1, L2, L3, prec, nrhs, ngroups;
+  std::vector<int> group_sizes;  // RHS per kappa group, in group order
   bool operator<(const TuneKey &o) const {
-    return std::tie(L0, L1, L2, L3, prec, nrhs, ngroups) < std::tie(o.L0, o.L1, o.L2, o.L3, o.prec, o.nrhs, o.ngroups);
+    return std::tie(L0, L1, L2, L3, prec, nrhs, ngroups, group_sizes) <
+           std::tie(o.L0, o.L1, o.L2, o.L3, o.prec, o.nrhs, o.ngroups, o.group_sizes);
   }
 };
+
+// The RHS of each kappa group in blocks of `block`, one group at a time: a
+// block never mixes kappa values.  Systems of one kappa converge at about the
+// same iteration count, while a lighter mass takes several times the
+// iterations of a heavier one (24^3x48: ~830 / 257 / 107 at kappa 0.1355 /
+// 0.13 / 0.12), and a block runs until its slowest member is done with the
+// converged ones masked out but still occupying their lanes.
+std::vector<std::vector<int>> plan_blocks(const MultiPlan &plan, int nrhs, int block) {
+  std::vector<std::vector<int>> blocks;
+  for (int gi = 0; gi < std::max(plan.ngroups, 1); gi++) {
+    std::vector<int> cur;
+    for (int k = 0; k < nrhs; k++) {
+      if (plan.ngroups > 0 && plan.group_of[k] != gi) continue;
+      cur.push_back(k);
+      if ((int)cur.size() == block) {
+        blocks.push_back(cur);
+        cur.clear();
+      }
+    }
+    if (!cur.empty()) blocks.push_back(cur);
+  }
+  return blocks;
+}
 static std::map<TuneKey, std::pair<int, int>> g_tuned;  // (block, tile)
 static std::mutex g_tuned_mutex;                        // contexts may be used from several threads
 
@@ -749,7 +769,9 @@ int cg_multi_prec(Context *c, MultiPlan &plan, int op, int nrhs, const double *k
   static const int tiles[2] = {kMinTile, 256};
   int tile = c->user_tile;
   if (block == 0) {
-    TuneKey key{c->g.L[0], c->g.L[1], c->g.L[2], c->g.L[3], prec, nrhs, plan.ngroups};
+    TuneKey key{c->g.L[0], c->g.L[1], c->g.L[2], c->g.L[3], prec, nrhs, plan.ngroups, {}};
+    for (int gi = 0; gi < plan.ngroups; gi++)
+      key.group_sizes.push_back((int)std::count(plan.group_of.begin(), plan.group_of.end(), gi));
     bool cached = false;
     {
       std::lock_guard<std::mutex> lock(g_tuned_mutex);
@@ -778,10 +800,10 @@ int cg_multi_prec(Context *c, MultiPlan &plan, int op, int nrhs, const double *k
           }
           c->multi_tile = 256;
           HB_TRY(r);
-          // the partial last block costs as much as a full one; the smaller
-          // workgroup must win by 3 % (a single matvec favours it slightly over
-          // what whole solves show)
-          const int nblk = (nrhs + nr - 1) / nr;
+          // a partial block (the last of a kappa group) costs as much as a
+          // full one; the smaller workgroup must win by 3 % (a single matvec
+          // favours it slightly over what whole solves show)
+          const int nblk = (int)plan_blocks(plan, nrhs, nr).size();
           const double total = ms * nr * nblk * (tl < 256 ? 1.03 : 1.0);
           if (total < best) {
             best = total;
@@ -804,9 +826,8 @@ int cg_multi_prec(Context *c, MultiPlan &plan, int op, int nrhs, const double *k
   c->last_tile = tile;
   c->multi_tile = tile;
   int result = HB_OK;
-  for (int k0 = 0; k0 < nrhs && result == HB_OK; k0 += block) {
-    std::vector<int> idx;
-    for (int k = k0; k < std::min(nrhs, k0 + block); k++) idx.push_back(k);
+  for (const std::vector<int> &idx : plan_blocks(plan, nrhs, block)) {
+    if (result != HB_OK) break;
     int r;
     switch (block) {
       case 1: r = cg_multi_block<S, G, 1>(c, plan, f, op, idx, kappa, rho, x, b, rest, maxiter, delta, max_inner, iters, true_rtr, nullptr); break;

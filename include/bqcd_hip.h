@@ -124,8 +124,9 @@ int hb_cg_mixed(hb_context *ctx, int op, double kappa, double rho, double *x, co
  * one system per cg() call): nrhs systems op(kappa[k], rho[k]) x[k] = b[k]
  * with host spinors in BQCD layout.  A block of right-hand sides shares one
  * pass of the block hopping matrix (every link and clover word read once for
- * the block); systems with different kappa (Hasenbusch shifts) can share a
- * block; converged systems are masked out.  csw > 0 builds the clover term of
+ * the block); the systems of each distinct kappa are blocked together (rho,
+ * e.g. a Hasenbusch shift, may differ within a block) and converged systems
+ * are masked out.  csw > 0 builds the clover term of
  * every distinct kappa on the device from the loaded links, csw == 0 means no
  * clover term, csw < 0 uses the loaded clover field (one kappa only).  Inner
  * iterations in precision (HB_PREC_F32 or HB_PREC_F16), FP64 reliable updates
