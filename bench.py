@@ -30,6 +30,13 @@ REPO = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(REPO, "bqcd-r399-quda_amd"))
 
 DSLASH_BYTES_PER_SITE = 8 * 144 + 192 + 192
+# one FP64 CG iteration on M~^+M~ (cg.F90:93-117) per even site: four fused
+# operator kernels (A^-1 D twice: 2112 B each; A x - k^2 D t: 2304 B; the same
+# with the dot product against p: 2496 B) and the vector updates (x, r:
+# 1152 B; p: 576 B); with two-row links each operator kernel reads 8 x 96 B
+# of links instead of 8 x 144 B
+CG_ITER_BYTES_PER_SITE_18 = 2 * 2112 + 2304 + 2496 + 1152 + 576
+CG_ITER_BYTES_PER_SITE_12 = CG_ITER_BYTES_PER_SITE_18 - 4 * 8 * 48
 # 16-bit hopping kernel with the A^-1 epilogue (first half of the mixed solver's
 # operator): 8 two-row links of 24 B, the input and output spinors (48 B + 4 B
 # norm each), two clover blocks of 18 words (72 B) plus their (m, c) float2
@@ -529,6 +536,10 @@ def main():
     del out, inp
 
     # ---- solves ---------------------------------------------------------------
+    # the FP64 CG iterates with two-row links (row 2 rebuilt; the smeared links
+    # are SU(3)); the 18-real form is timed once beside it
+    g.set_link_recon(12)
+
     def solve_f64():
         x.zero_()
         it, tr = g.cg_d("mtdagmt", x, b, kappa, rest=rest, maxiter=20000)
@@ -564,6 +575,15 @@ def main():
         fp64 = {"ms": t0.elapsed_time(t1), "iters": it64, "true_rel": tr64 / bnorm2, "x": x.clone()}
         log("FP64 solve: %.1f ms, %d iterations, true |r|^2/|b|^2 = %.3e"
             % (fp64["ms"], it64, tr64 / bnorm2))
+        g.set_link_recon(18)
+        t0.record(stream)
+        it18, _, tr18 = solve_f64()
+        t1.record(stream)
+        torch.cuda.synchronize()
+        g.set_link_recon(12)
+        fp64["ms18"], fp64["iters18"], fp64["true_rel18"] = t0.elapsed_time(t1), it18, tr18 / bnorm2
+        log("FP64 solve, 18-real links: %.1f ms, %d iterations, true |r|^2/|b|^2 = %.3e"
+            % (fp64["ms18"], it18, tr18 / bnorm2))
 
     for _ in range(args.warmup):
         it, nrel, tr = solve()
@@ -596,6 +616,8 @@ def main():
         failures.append("%s solve: true |r|^2/|b|^2 = %.3e > 1e-20" % (args.solver, true_rel))
     if fp64 and fp64["true_rel"] > 1e-20:
         failures.append("FP64 solve: true |r|^2/|b|^2 = %.3e > 1e-20" % fp64["true_rel"])
+    if fp64 and fp64["true_rel18"] > 1e-20:
+        failures.append("FP64 solve (18-real links): true |r|^2/|b|^2 = %.3e > 1e-20" % fp64["true_rel18"])
 
     # the dominant kernel of the mixed solve: the 16-bit hopping kernel, timed in
     # isolation with HIP events on the context stream
@@ -660,7 +682,7 @@ def main():
             failures.append("batched solve: true |r|^2/|b|^2 = %.3e > 1e-20"
                             % batch["max_true_residual_rel"])
 
-    cg_bytes = 10560 * vh * (fp64.get("iters") or iters[0])
+    cg_bytes = CG_ITER_BYTES_PER_SITE_12 * vh * (fp64.get("iters") or iters[0])
     traffic, traffic_src = measured_traffic(L)
     fp64_ms = fp64.get("ms", ms)
     line = {
@@ -688,6 +710,12 @@ def main():
         "fp64_cg_iterations": fp64.get("iters", iters[0]),
         "speedup_vs_fp64": fp64_ms / ms,
         "fp64_cg_algorithmic_GBs": cg_bytes / (fp64_ms * 1e-3) / 1e9,
+        "fp64_cg_link_reals": 12,
+        "fp64_cg_ms_per_iteration": fp64_ms / (fp64.get("iters") or iters[0]),
+        "fp64_cg_18real_s": fp64.get("ms18", float("nan")) / 1e3,
+        "fp64_cg_18real_iterations": fp64.get("iters18"),
+        "fp64_cg_18real_algorithmic_GBs": (CG_ITER_BYTES_PER_SITE_18 * vh * fp64["iters18"]
+                                           / (fp64["ms18"] * 1e-3) / 1e9) if fp64 else None,
         "dslash_fp64_ms": ds_ms,
         "dslash_fp64_GBs": ds_gbs,
         "dslash_fp64_GFlops": DSLASH_FLOPS_PER_SITE * vh / (ds_ms * 1e-3) / 1e9,

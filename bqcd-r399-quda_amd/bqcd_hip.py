@@ -113,6 +113,7 @@ def lib():
         L.hb_apply_d.argtypes = [_vp, ctypes.c_int, ctypes.c_double, ctypes.c_double, _vp, _vp]
         L.hb_cg_d.argtypes = [_vp, ctypes.c_int, ctypes.c_double, ctypes.c_double, _vp, _vp,
                               ctypes.c_float, ctypes.c_int, ctypes.c_int, _ip, _dp]
+        L.hb_set_link_recon.argtypes = [_vp, ctypes.c_int]
         L.hb_cg_d_noalloc.argtypes = [_vp, ctypes.c_int, ctypes.c_double, ctypes.c_double, _vp, _vp,
                                       ctypes.c_float, ctypes.c_int, ctypes.c_int, _ip, _dp]
         L.hb_spinor_to_device.argtypes = [_vp, _vp, _dp]
@@ -470,6 +471,11 @@ class Lattice:
         _check(lib().hb_cg_d(self._h, OP[op], kappa, rho, self._s(x), self._s(b), ctypes.c_float(rest),
                              maxiter, log, ctypes.byref(it), ctypes.byref(tr)))
         return it.value, tr.value
+
+    def set_link_recon(self, recon):
+        """links the FP64 CG iterates with: 12 (rows 0 and 1, row 2 rebuilt; the
+        default, used while the links are SU(3)) or 18"""
+        _check(lib().hb_set_link_recon(self._h, recon))
 
     def cg_d_noalloc(self, op, x, b, kappa, rho=0.0, rest=1e-20, maxiter=1000, log=0):
         """cg_d with the library's static work area: allocates no device memory

@@ -9,10 +9,11 @@ namespace hb {
 // that an iteration needs no host round trip
 struct CgState {
   double rtr_old, rtr, pap, beta;
+  double alpha;    // step of the iteration whose x update is pending
   double rest;     // stopping bound on |r|^2 (cg_para%rest, promoted to FP64)
   int iter, maxiter;
   int done;        // 0 running, 1 converged, 2 maxiter reached
-  int pad;
+  int xpend;       // 1: x += alpha p of the last iteration is still to be done
   double res[4];   // scratch results (true residual etc.)
 };
 
@@ -26,10 +27,19 @@ struct FinCgStart {  // rtr_old = |r|^2
     st->done = 0;
   }
 };
-struct FinCgUpdate {  // rtr = |r|^2; stop test (cg.F90:111); beta = rtr / rtr_old
+// rtr = |r|^2; stop test (cg.F90:111); beta = rtr / rtr_old.  Launched
+// without a skip flag: once the solve is done it only clears xpend, so that
+// the x update of the last iteration (in k_cg_pupdate) happens exactly once.
+struct FinCgUpdate {
   CgState *st;
   __device__ void operator()(const double (&t)[1]) const {
+    if (st->done) {
+      st->xpend = 0;
+      return;
+    }
     const double rtr = t[0];
+    st->alpha = st->rtr_old / st->pap;
+    st->xpend = 1;
     st->iter += 1;
     st->rtr = rtr;
     if (rtr <= st->rest) {
@@ -57,9 +67,8 @@ int clover_mult_f64(Context *c, int which, int eo, double2 *out, const double2 *
 // doubles); null selects the context's own buffer
 int cg_init(Context *c, CgState *st, double rest, int maxiter);
 int cg_start(Context *c, double2 *r, double2 *p, const double2 *b, CgState *st, double *red = nullptr);
-int cg_update(Context *c, double2 *x, double2 *r, const double2 *p, const double2 *q, CgState *st,
-              double *red = nullptr);
-int cg_pupdate(Context *c, double2 *p, const double2 *r, const CgState *st);
+int cg_update(Context *c, double2 *r, const double2 *q, CgState *st, double *red = nullptr);
+int cg_pupdate(Context *c, double2 *x, double2 *p, const double2 *r, const CgState *st);
 int residual(Context *c, double2 *out, const double2 *b, double *res_d, double *red = nullptr);
 int dot_norm(Context *c, const double2 *x, const double2 *y, double *res_d);
 int zero_f64(Context *c, double2 *p, size_t n);

@@ -38,7 +38,14 @@ int clover_mult_f64(Context *c, int which, int eo, double2 *out, const double2 *
 #define HB_SITE_LOOP for (int k = 0; k < 12; k++)
 #define HB_AT(ptr, k) (ptr)[(size_t)(k) * vh + i]
 // streaming (non-temporal) store of one complex number, see store_spinor
+#ifndef HB_CG_NT_STORE
+#define HB_CG_NT_STORE 1
+#endif
+#if HB_CG_NT_STORE
 #define HB_ST(ptr, k, v) SF64::st_nt((ptr), (size_t)(k) * vh + i, (v))
+#else
+#define HB_ST(ptr, k, v) ((ptr)[(size_t)(k) * vh + i] = (v))
+#endif
 
 // r = b - r, p = r, rtr_old = |r|^2  (cg.F90:84-89)
 __global__ __launch_bounds__(kBlock) void k_cg_start(double2 *__restrict__ r, double2 *__restrict__ p,
@@ -60,11 +67,10 @@ __global__ __launch_bounds__(kBlock) void k_cg_start(double2 *__restrict__ r, do
   block_partials<1>(vals, partials);
 }
 
-// alpha = rtr_old / pAp; x += alpha p; r -= alpha q; rtr = |r|^2 (cg.F90:96-109)
-__global__ __launch_bounds__(kBlock) void k_cg_update(double2 *__restrict__ x, double2 *__restrict__ r,
-                                                      const double2 *__restrict__ p,
-                                                      const double2 *__restrict__ q, int vh,
-                                                      const CgState *st, double *partials) {
+// alpha = rtr_old / pAp; r -= alpha q; rtr = |r|^2 (cg.F90:96-109).  The
+// x update of the same iteration is done by k_cg_pupdate, which reads p anyway.
+__global__ __launch_bounds__(kBlock) void k_cg_update(double2 *__restrict__ r, const double2 *__restrict__ q,
+                                                      int vh, const CgState *st, double *partials) {
   if (st->done) return;
   const double alpha = st->rtr_old / st->pap;
   int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -72,10 +78,8 @@ __global__ __launch_bounds__(kBlock) void k_cg_update(double2 *__restrict__ x, d
   if (i < vh) {
 #pragma unroll
     HB_SITE_LOOP {
-      double2 pp = HB_AT(p, k), qq = HB_AT(q, k), xx = HB_AT(x, k), rr = HB_AT(r, k);
-      xx.x += alpha * pp.x; xx.y += alpha * pp.y;
+      double2 qq = HB_AT(q, k), rr = HB_AT(r, k);
       rr.x -= alpha * qq.x; rr.y -= alpha * qq.y;
-      HB_ST(x, k, xx);
       HB_ST(r, k, rr);
       s += rr.x * rr.x + rr.y * rr.y;
     }
@@ -84,17 +88,26 @@ __global__ __launch_bounds__(kBlock) void k_cg_update(double2 *__restrict__ x, d
   block_partials<1>(vals, partials);
 }
 
-// p = r + beta p  (sc_xpby, sc.F90:184-201)
-__global__ __launch_bounds__(kBlock) void k_cg_pupdate(double2 *__restrict__ p, const double2 *__restrict__ r,
-                                                       int vh, const CgState *st) {
-  if (st->done) return;
-  const double beta = st->beta;
+// x += alpha p of the iteration just finished (cg.F90:97-98), then, unless the
+// solve has ended, p = r + beta p (sc_xpby, sc.F90:184-201); the same
+// arithmetic on the same values as updating x beside r
+__global__ __launch_bounds__(kBlock) void k_cg_pupdate(double2 *__restrict__ x, double2 *__restrict__ p,
+                                                       const double2 *__restrict__ r, int vh,
+                                                       const CgState *st) {
+  if (!st->xpend) return;
+  const double alpha = st->alpha, beta = st->beta;
+  const bool next = st->done == 0;
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= vh) return;
 #pragma unroll
   HB_SITE_LOOP {
-    double2 pp = HB_AT(p, k), rr = HB_AT(r, k);
-    HB_ST(p, k, make_double2(rr.x + beta * pp.x, rr.y + beta * pp.y));
+    double2 pp = HB_AT(p, k), xx = HB_AT(x, k);
+    xx.x += alpha * pp.x; xx.y += alpha * pp.y;
+    HB_ST(x, k, xx);
+    if (next) {
+      double2 rr = HB_AT(r, k);
+      HB_ST(p, k, make_double2(rr.x + beta * pp.x, rr.y + beta * pp.y));
+    }
   }
 }
 
@@ -177,19 +190,18 @@ int cg_start(Context *c, double2 *r, double2 *p, const double2 *b, CgState *st, 
   return HB_OK;
 }
 
-int cg_update(Context *c, double2 *x, double2 *r, const double2 *p, const double2 *q, CgState *st,
-              double *red) {
+int cg_update(Context *c, double2 *r, const double2 *q, CgState *st, double *red) {
   const int vh = c->g.vh, grid = nblocks(vh);
   if (!red) red = c->red;
-  k_cg_update<<<grid, kBlock, 0, c->stream>>>(x, r, p, q, vh, st, red);
-  k_finalize<1><<<1, kFinThreads, 0, c->stream>>>(red, grid, &st->done, FinCgUpdate{st});
+  k_cg_update<<<grid, kBlock, 0, c->stream>>>(r, q, vh, st, red);
+  k_finalize<1><<<1, kFinThreads, 0, c->stream>>>(red, grid, nullptr, FinCgUpdate{st});
   HB_HIP(hipGetLastError());
   return HB_OK;
 }
 
-int cg_pupdate(Context *c, double2 *p, const double2 *r, const CgState *st) {
+int cg_pupdate(Context *c, double2 *x, double2 *p, const double2 *r, const CgState *st) {
   const int vh = c->g.vh;
-  k_cg_pupdate<<<nblocks(vh), kBlock, 0, c->stream>>>(p, r, vh, st);
+  k_cg_pupdate<<<nblocks(vh), kBlock, 0, c->stream>>>(x, p, r, vh, st);
   HB_HIP(hipGetLastError());
   return HB_OK;
 }

@@ -2,6 +2,7 @@
 // from the fused hopping-matrix kernels, and the FP64 CG (cg.F90:51-137).
 #include <chrono>
 #include <cmath>
+#include <cstring>
 
 #include "hb_blas.h"
 #include "hb_dslash.h"
@@ -24,15 +25,27 @@ DslashArgs<SF64> dslash_args64(Context *c, int out_parity, double2 *out, const d
 
 // half of M~ (or M~^+): t = A_oo^-1 D_oe in ; out = A_ee in - kappa^2 D_eo t + rho in
 // (m_tilde.F90:61-71 / :98-108 with w_mult.F90's sc_axpy folded in).  With
-// dot_out set, also Re<dotv, out> (sc_dot) in the same pass.
-int mtil_f64(Context *c, bool dag, double kappa, double rho, double2 *out, const double2 *in,
-             double2 *t, double *dot_out, const double2 *dotv, const int *skip, double *red) {
+// dot_out set, also Re<dotv, out> (sc_dot) in the same pass.  G = SF64 reads
+// all 18 reals of each link, G = SF64R the two-row copy U (row 2 rebuilt).
+template <class G>
+int mtil_f64_g(Context *c, const typename G::ST *U, bool dag, double kappa, double rho, double2 *out,
+               const double2 *in, double2 *t, double *dot_out, const double2 *dotv, const int *skip,
+               double *red) {
   const bool clov = use_clover(c);
-  DslashArgs<SF64> a1 = dslash_args64(c, 1, t, in);
+  DslashArgs<SF64, G> a1{};
+  a1.g = c->g;
+  a1.out_parity = 1;
+  a1.out = {t, nullptr};
+  a1.in = {const_cast<double2 *>(in), nullptr};
+  a1.U = U;
+  a1.partials = c->red;
   a1.C = clov ? c->clov[HB_CLOVER_AINV] : nullptr;
   a1.skip = skip;
   HB_TRY(launch_dslash(a1, dag, EPI_CLINV, c->stream));
-  DslashArgs<SF64> a2 = dslash_args64(c, 0, out, t);
+  DslashArgs<SF64, G> a2 = a1;
+  a2.out_parity = 0;
+  a2.out = {out, nullptr};
+  a2.in = {t, nullptr};
   a2.C = clov ? c->clov[HB_CLOVER_A] : nullptr;
   a2.x0 = {const_cast<double2 *>(in), nullptr};
   a2.mkappa2 = -kappa * kappa;
@@ -49,18 +62,81 @@ int mtil_f64(Context *c, bool dag, double kappa, double rho, double2 *out, const
   return HB_OK;
 }
 
+int mtil_f64(Context *c, bool dag, double kappa, double rho, double2 *out, const double2 *in,
+             double2 *t, double *dot_out, const double2 *dotv, const int *skip, double *red, bool recon) {
+  if (recon)
+    return mtil_f64_g<SF64R>(c, c->gauge64r, dag, kappa, rho, out, in, t, dot_out, dotv, skip, red);
+  return mtil_f64_g<SF64>(c, c->gauge, dag, kappa, rho, out, in, t, dot_out, dotv, skip, red);
+}
+
+// rows 0 and 1 of the FP64 links; the largest |row 2 - conj(row0 x row1)| of
+// any link goes to *dev (as float bits, non-negative: integer max = float max)
+__global__ __launch_bounds__(kBlock) void k_links_two_rows(double2 *__restrict__ dst,
+                                                           const double2 *__restrict__ src, int vh,
+                                                           unsigned int *dev) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int slab = blockIdx.y;  // mu*2 + eo
+  float m = 0.f;
+  if (i < vh) {
+    const double2 *s = src + (size_t)slab * 9 * vh + i;
+    double2 u[9];
+#pragma unroll
+    for (int k = 0; k < 9; k++) u[k] = s[(size_t)k * vh];
+#pragma unroll
+    for (int k = 0; k < 6; k++) dst[((size_t)slab * 6 + k) * vh + i] = u[k];
+    auto cross = [](double2 a0, double2 b1, double2 a1, double2 b0) {  // conj(a0 b1 - a1 b0)
+      double2 t = cmul(a0, b1), q = cmul(a1, b0);
+      return make_double2(t.x - q.x, -(t.y - q.y));
+    };
+    const double2 r[3] = {cross(u[1], u[5], u[2], u[4]), cross(u[2], u[3], u[0], u[5]),
+                          cross(u[0], u[4], u[1], u[3])};
+#pragma unroll
+    for (int k = 0; k < 3; k++)
+      m = fmaxf(m, (float)fmax(fabs(r[k].x - u[6 + k].x), fabs(r[k].y - u[6 + k].y)));
+  }
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) m = fmaxf(m, __shfl_xor(m, o, 64));
+  if ((threadIdx.x & 63) == 0) atomicMax(dev, __float_as_uint(m));
+}
+
+// Brings the two-row copy of the links up to date (allocating it on first
+// use).  Returns in *use whether the FP64 CG may iterate with it: the context
+// asks for 12-real links and every link is SU(3) to 1e-12, so that the
+// rebuilt row 2 equals the stored one to rounding.
+int ensure_links_two_rows(Context *c, bool *use) {
+  *use = false;
+  if (c->link_recon != 12) return HB_OK;
+  if (c->r64_gauge_version != c->gauge_version) {
+    const size_t vh = c->g.vh;
+    if (!c->gauge64r) HB_HIP(hipMalloc(&c->gauge64r, sizeof(double2) * 48 * vh));
+    unsigned int *dev = reinterpret_cast<unsigned int *>(c->red);
+    HB_TRY(zero_f64(c, reinterpret_cast<double2 *>(c->red), 1));
+    k_links_two_rows<<<dim3(nblocks((int)vh), 8), kBlock, 0, c->stream>>>(c->gauge64r, c->gauge, (int)vh, dev);
+    HB_HIP(hipGetLastError());
+    unsigned int bits = 0;
+    HB_HIP(hipMemcpyAsync(&bits, dev, sizeof(bits), hipMemcpyDeviceToHost, c->stream));
+    HB_HIP(hipStreamSynchronize(c->stream));
+    float d = 0.f;
+    std::memcpy(&d, &bits, sizeof(d));
+    c->r64_ok = d <= 1e-12f;
+    c->r64_gauge_version = c->gauge_version;
+  }
+  *use = c->r64_ok;
+  return HB_OK;
+}
+
 int apply_f64(Context *c, int op, double kappa, double rho, double2 *out, const double2 *in,
-              double2 *t, double2 *y, double *dot_out, const int *skip, double *red) {
+              double2 *t, double2 *y, double *dot_out, const int *skip, double *red, bool recon) {
   switch (op) {
-    case HB_OP_MTIL: return mtil_f64(c, false, kappa, 0.0, out, in, t, nullptr, nullptr, skip);
-    case HB_OP_MTIL_DAG: return mtil_f64(c, true, kappa, 0.0, out, in, t, nullptr, nullptr, skip);
-    case HB_OP_W: return mtil_f64(c, false, kappa, rho, out, in, t, nullptr, nullptr, skip);
-    case HB_OP_W_DAG: return mtil_f64(c, true, kappa, rho, out, in, t, nullptr, nullptr, skip);
+    case HB_OP_MTIL: return mtil_f64(c, false, kappa, 0.0, out, in, t, nullptr, nullptr, skip, nullptr, recon);
+    case HB_OP_MTIL_DAG: return mtil_f64(c, true, kappa, 0.0, out, in, t, nullptr, nullptr, skip, nullptr, recon);
+    case HB_OP_W: return mtil_f64(c, false, kappa, rho, out, in, t, nullptr, nullptr, skip, nullptr, recon);
+    case HB_OP_W_DAG: return mtil_f64(c, true, kappa, rho, out, in, t, nullptr, nullptr, skip, nullptr, recon);
     case HB_OP_MTDAGMT:
     case HB_OP_WDAGW: {
       double r = op == HB_OP_WDAGW ? rho : 0.0;
-      HB_TRY(mtil_f64(c, false, kappa, r, y, in, t, nullptr, nullptr, skip));
-      return mtil_f64(c, true, kappa, r, out, y, t, dot_out, in, skip, red);
+      HB_TRY(mtil_f64(c, false, kappa, r, y, in, t, nullptr, nullptr, skip, nullptr, recon));
+      return mtil_f64(c, true, kappa, r, out, y, t, dot_out, in, skip, red, recon);
     }
   }
   set_error("unknown operator %d", op);
@@ -91,7 +167,7 @@ int cg_f64_run(Context *c, const CgWork &w, int op, double kappa, double rho, do
   CgState *st = w.st, *hs = w.hs;
   HB_TRY(cg_init(c, st, (double)rest, maxiter));
   // r = A x; r = b - r; p = r; rtr_old = |r|^2
-  HB_TRY(apply_f64(c, op, kappa, rho, w.r, x, w.t, w.y, nullptr, nullptr));
+  HB_TRY(apply_f64(c, op, kappa, rho, w.r, x, w.t, w.y, nullptr, nullptr, nullptr, w.recon));
   HB_TRY(cg_start(c, w.r, w.p, b, st, w.red));
   int *skip = &st->done;
   int nmv = 1;
@@ -105,9 +181,9 @@ int cg_f64_run(Context *c, const CgWork &w, int op, double kappa, double rho, do
     int chunk = 8;
     for (;;) {
       for (int k = 0; k < chunk; k++) {
-        HB_TRY(apply_f64(c, op, kappa, rho, w.q, w.p, w.t, w.y, &st->pap, skip, w.red));
-        HB_TRY(cg_update(c, x, w.r, w.p, w.q, st, w.red));
-        HB_TRY(cg_pupdate(c, w.p, w.r, st));
+        HB_TRY(apply_f64(c, op, kappa, rho, w.q, w.p, w.t, w.y, &st->pap, skip, w.red, w.recon));
+        HB_TRY(cg_update(c, w.r, w.q, st, w.red));
+        HB_TRY(cg_pupdate(c, x, w.p, w.r, st));
       }
       HB_HIP(hipMemcpyAsync(hs, st, sizeof(CgState), hipMemcpyDeviceToHost, c->stream));
       HB_HIP(hipStreamSynchronize(c->stream));
@@ -119,7 +195,8 @@ int cg_f64_run(Context *c, const CgWork &w, int op, double kappa, double rho, do
   const int niter = hs->iter;
   const int done = hs->done;
   const double rtr = hs->rtr;
-  // true residual |b - A x|^2 in FP64 (invert_quda.F90:155-168)
+  // true residual |b - A x|^2 in FP64 (invert_quda.F90:155-168), with all
+  // 18 reals of each link
   HB_TRY(apply_f64(c, op, kappa, rho, w.q, x, w.t, w.y, nullptr, nullptr));
   HB_TRY(residual(c, w.q, b, &st->res[0], w.red));
   HB_HIP(hipMemcpyAsync(hs, st, sizeof(CgState), hipMemcpyDeviceToHost, c->stream));
@@ -148,6 +225,7 @@ int cg_f64(Context *c, int op, double kappa, double rho, double2 *x, const doubl
   w.red = c->red;
   HB_TRY(ensure_host_stage(c, sizeof(CgState)));
   w.hs = static_cast<CgState *>(c->host_stage);
+  HB_TRY(ensure_links_two_rows(c, &w.recon));
 
   EventTimer tm;
   HB_TRY(tm.create());

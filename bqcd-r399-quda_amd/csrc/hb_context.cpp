@@ -126,6 +126,7 @@ int hb_finalize(hb_context *ctx) {
   if (c->own_stream) (void)hipStreamSynchronize(c->own_stream);
   (void)hipFree(c->gauge);
   for (auto *p : c->clov) (void)hipFree(p);
+  (void)hipFree(c->gauge64r);
   (void)hipFree(c->gauge32);
   for (auto *p : c->clov32) (void)hipFree(p);
   (void)hipFree(c->gauge16);
@@ -148,6 +149,14 @@ int hb_set_stream(hb_context *ctx, void *stream) {
   Context *c = reinterpret_cast<Context *>(ctx);
   HB_CHECK(c, HB_E_ARG, "hb_set_stream: null context");
   c->stream = stream ? static_cast<hipStream_t>(stream) : c->own_stream;
+  return HB_OK;
+}
+
+int hb_set_link_recon(hb_context *ctx, int recon) {
+  Context *c = reinterpret_cast<Context *>(ctx);
+  HB_CHECK(c, HB_E_ARG, "hb_set_link_recon: null context");
+  HB_CHECK(recon == 12 || recon == 18, HB_E_ARG, "hb_set_link_recon: %d (use 12 or 18)", recon);
+  c->link_recon = recon;
   return HB_OK;
 }
 

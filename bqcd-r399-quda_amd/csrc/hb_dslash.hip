@@ -118,6 +118,8 @@ int launch_dslash(const DslashArgs<S, G> &a, bool dag, int epi, hipStream_t st) 
 }
 
 template int launch_dslash<SF64, SF64>(const DslashArgs<SF64, SF64> &, bool, int, hipStream_t);
+// FP64 spinors with two-row FP64 links (the FP64 CG iteration)
+template int launch_dslash<SF64, SF64R>(const DslashArgs<SF64, SF64R> &, bool, int, hipStream_t);
 template int launch_dslash<SF32, SF32>(const DslashArgs<SF32, SF32> &, bool, int, hipStream_t);
 template int launch_dslash<SF16, SF16>(const DslashArgs<SF16, SF16> &, bool, int, hipStream_t);
 // FP32 spinors with FP16 links and clover

@@ -137,6 +137,11 @@ struct Context {
   float2 *clov32[2] = {nullptr, nullptr};          // A, A^-1
   short2 *gauge16 = nullptr;                       // 16-bit fixed point
   short2 *clov16[2] = {nullptr, nullptr};          // words and (scale, diagonal offset) per block
+  // FP64 links as rows 0 and 1 for the FP64 CG iteration (built on demand)
+  double2 *gauge64r = nullptr;                     // [4][2][6][vh]
+  uint64_t r64_gauge_version = 0;                  // gauge_version it was built from
+  bool r64_ok = false;                             // the links are SU(3) to 1e-12
+  int link_recon = 12;                             // links of the FP64 CG iteration: 12 or 18 reals
   uint64_t gauge_version = 1, clover_version = 1;
   uint64_t lp_gauge_version = 0, lp_clover_version = 0;
   // scratch

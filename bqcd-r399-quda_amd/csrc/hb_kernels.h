@@ -68,6 +68,12 @@ struct SF64 {
   __device__ __forceinline__ static V bld(rsrc_t r, uint32_t vo, uint32_t so) { return bld_f64x2<POL>(r, vo, so); }
 };
 
+// FP64 links stored as rows 0 and 1 ([4][2][6][vh]); row 2 is rebuilt as
+// conj(row0 x row1), exact for SU(3) links (the FP64 solvers' operator)
+struct SF64R : SF64 {
+  static constexpr bool kRecon = true;
+};
+
 struct SF32 {
   using A = float;
   using V = float2;

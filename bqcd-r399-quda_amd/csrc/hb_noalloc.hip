@@ -1,7 +1,9 @@
 // hb_noalloc.hip - the FP64 CG of hb_cg_d with every work field in one
 // statically allocated device array, for callers that cannot allocate device
 // memory (code under stream capture, tight solver loops, harnesses that
-// forbid hipMalloc).  The kernels and the iteration are those of hb_cg_d.
+// forbid hipMalloc).  The kernels and the iteration are those of hb_cg_d with
+// all 18 reals of each link read (the two-row copy hb_cg_d may iterate with is
+// allocated on demand, so this entry point does not use it).
 //
 // The array holds, at fixed offsets, the five work spinors of the CG (the
 // operator temporaries t and y, and r, p, q), the device CG state and the

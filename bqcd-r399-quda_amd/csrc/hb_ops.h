@@ -69,11 +69,15 @@ int check_ready(const Context *c);
 size_t spinor_bytes(const Context *c);
 double2 *spinor_scratch(Context *c, int slot, int *rc);
 DslashArgs<SF64> dslash_args64(Context *c, int out_parity, double2 *out, const double2 *in);
-// `red` (optional): partials buffer of the fused dot product, else the context's
+// `red` (optional): partials buffer of the fused dot product, else the context's;
+// `recon`: iterate with the two-row copy of the links (ensure_links_two_rows)
 int mtil_f64(Context *c, bool dag, double kappa, double rho, double2 *out, const double2 *in,
-             double2 *t, double *dot_out, const double2 *dotv, const int *skip, double *red = nullptr);
+             double2 *t, double *dot_out, const double2 *dotv, const int *skip, double *red = nullptr,
+             bool recon = false);
 int apply_f64(Context *c, int op, double kappa, double rho, double2 *out, const double2 *in,
-              double2 *t, double2 *y, double *dot_out, const int *skip, double *red = nullptr);
+              double2 *t, double2 *y, double *dot_out, const int *skip, double *red = nullptr,
+              bool recon = false);
+int ensure_links_two_rows(Context *c, bool *use);
 
 // work fields of the FP64 CG: operator temporaries t (odd) and y (even), the
 // CG vectors r, p, q = A p, the device state, the reduction partials (at
@@ -83,6 +87,7 @@ struct CgWork {
   CgState *st = nullptr;
   double *red = nullptr;
   CgState *hs = nullptr;
+  bool recon = false;  // iterate with the two-row links (the true residual uses all 18 reals)
 };
 int cg_f64_run(Context *c, const CgWork &w, int op, double kappa, double rho, double2 *x,
                const double2 *b, float rest, int maxiter, int log, int *iters, double *true_rtr,

@@ -181,10 +181,19 @@ int hb_apply_d(hb_context *ctx, int op, double kappa, double rho, double *out_d,
                const double *in_d);
 int hb_cg_d(hb_context *ctx, int op, double kappa, double rho, double *x_d, const double *b_d,
             float rest, int maxiter, int log, int *iters, double *true_rtr);
+/* Links the FP64 CG iterates with: 12 (default; rows 0 and 1 are read and row 2
+ * is rebuilt as conj(row0 x row1), the reconstruction BQCD asked of QUDA,
+ * invert_quda.F90:234) or 18 (every stored real).  The 12-real form is used
+ * only while every loaded link is SU(3) to 1e-12, otherwise the CG reads all
+ * 18.  The final true residual, hb_dslash, hb_apply and the forces always use
+ * all 18 reals.  No counterpart in BQCD's own cg(). */
+int hb_set_link_recon(hb_context *ctx, int recon);
+
 /* hb_cg_d without any allocation.  It has no counterpart in the reference; it
  * is for callers that cannot allocate device memory (code under stream
  * capture, tight solver loops, harnesses that forbid hipMalloc).  Arguments,
- * kernels and results are those of hb_cg_d.  Its work fields (five spinors,
+ * kernels and results are those of hb_cg_d with 18-real links
+ * (hb_set_link_recon(ctx, 18)).  Its work fields (five spinors,
  * the CG state, the reduction partials) live in one static device array per
  * GPU, sized for volh <= HB_NOALLOC_MAX_VOLH (60 MiB, held until the process
  * ends); a larger lattice returns HB_E_ARG.  Every call on one GPU uses that

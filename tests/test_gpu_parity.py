@@ -74,6 +74,47 @@ def test_cg_vs_reference(hip, golden, case):
     g.close()
 
 
+@pytest.mark.parametrize("case", ["l4444", "l4846", "l8888"])
+def test_cg_link_recon(hip, golden, case):
+    """The FP64 CG iterating with two-row links (row 2 rebuilt, the default for
+    SU(3) links) and with all 18 reals: both take the reference's iteration
+    counts and give its solution to 1e-12, and they agree to 1e-13.  Links
+    that are not SU(3) (scaled by 1 + 1e-6) make the CG fall back to all 18
+    reals: the two settings then give bit-identical results."""
+    f = golden(case)
+    lat, u, eta_e, eta_o = make_case(f)
+    kappa, rho = float(f["kappa"]), float(f["rho"])
+    rest, maxiter = float(f["rest"]), int(f["maxiter"])
+    g = hip.Lattice(lat.L)
+    g.load_gauge(u, flipped=False)
+    g.build_clover(float(f["csw"]) * kappa)
+    xs = {}
+    for recon in (12, 18):
+        g.set_link_recon(recon)
+        x, it, tr = g.cg("mtdagmt", eta_e, kappa, rest=rest, maxiter=maxiter)
+        assert it == int(f["info_cg_iters"]), recon
+        assert tr <= 2 * rest
+        x2, it2, _ = g.cg("w_dagger_w", eta_e, kappa, rho, rest=rest, maxiter=maxiter, x0=0.01 * eta_o)
+        assert it2 == int(f["info_cgw_iters"]), recon
+        if "cg_x" in f:
+            assert rel(x, f["cg_x"]) < 1e-12
+            assert rel(x2, f["cgw_x"]) < 1e-12
+        else:
+            sites = f["sites"]
+            assert rel(x[sites], f["cg_x_sample"]) < 1e-12
+            assert rel(x2[sites], f["cgw_x_sample"]) < 1e-12
+        xs[recon] = x
+    assert rel(xs[12], xs[18]) < 1e-13
+    # not SU(3): the two-row copy is refused, both settings read all 18 reals
+    g.load_gauge(u * (1.0 + 1e-6), flipped=False)
+    out = []
+    for recon in (12, 18):
+        g.set_link_recon(recon)
+        out.append(g.cg("mtdagmt", eta_e, kappa, rest=rest, maxiter=maxiter))
+    assert out[0][1] == out[1][1] and np.array_equal(out[0][0], out[1][0]) and out[0][2] == out[1][2]
+    g.close()
+
+
 @pytest.mark.parametrize("L", [(4, 4, 4, 8), (8, 4, 6, 4), (6, 6, 6, 6), (8, 8, 8, 16)])
 def test_ops_vs_oracle_shapes(hip, L):
     """ragged extents, every direction wrapping; antiperiodic time"""

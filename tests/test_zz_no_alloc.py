@@ -1,6 +1,7 @@
 """The allocation-free FP64 CG (hb_cg_d_noalloc) against hb_cg_d, through the
-C ABI.  Both run the same kernels with the same deterministic reductions, so
-the iteration counts, solutions and true residuals must be identical.
+C ABI.  Both run the same kernels with the same deterministic reductions (with
+all 18 reals of each link read, hb_set_link_recon 18), so the iteration
+counts, solutions and true residuals must be identical.
 
 Two calls of hb_cg_d_noalloc in a row share one static work area: first a
 16^3x32 lattice (the largest it takes, volh = HB_NOALLOC_MAX_VOLH) on
@@ -45,6 +46,7 @@ def test_noalloc_cg_matches_cg_d_on_two_calls():
     for L, seed, kappa, csw, op, rho, guess in cases:
         g, b, x0 = setup(hip, torch, dev, L, seed, kappa, csw, guess)
         assert g.volh <= MAX_VOLH
+        g.set_link_recon(18)
         rest = 1e-20 * float((b * b).sum())
         x = x0.clone()
         it, tr = g.cg_d(op, x, b, kappa, rho, rest=rest, maxiter=5000)
