@@ -32,10 +32,11 @@ sys.path.insert(0, os.path.join(REPO, "bqcd-r399-quda_amd"))
 DSLASH_BYTES_PER_SITE = 8 * 144 + 192 + 192
 # one FP64 CG iteration on M~^+M~ (cg.F90:93-117) per even site: four fused
 # operator kernels (A^-1 D twice: 2112 B each; A x - k^2 D t: 2304 B; the same
-# with the dot product against p: 2496 B) and the vector updates (x, r:
-# 1152 B; p: 576 B); with two-row links each operator kernel reads 8 x 96 B
-# of links instead of 8 x 144 B
-CG_ITER_BYTES_PER_SITE_18 = 2 * 2112 + 2304 + 2496 + 1152 + 576
+# with the dot product against p: 2496 B) and the vector updates (r with |r|^2:
+# reads q, r, writes r, 576 B; x and p: reads r, p, x, writes p, x, 960 B);
+# with two-row links each operator kernel reads 8 x 96 B of links instead of
+# 8 x 144 B
+CG_ITER_BYTES_PER_SITE_18 = 2 * 2112 + 2304 + 2496 + 576 + 960
 CG_ITER_BYTES_PER_SITE_12 = CG_ITER_BYTES_PER_SITE_18 - 4 * 8 * 48
 # 16-bit hopping kernel with the A^-1 epilogue (first half of the mixed solver's
 # operator): 8 two-row links of 24 B, the input and output spinors (48 B + 4 B
