@@ -78,6 +78,9 @@ def parse():
     ap.add_argument("--only-dslash", type=int, default=0,
                     help="run only N FP64 hopping-matrix launches and a calibration copy of the "
                          "same byte count (for hardware-counter runs), then exit")
+    ap.add_argument("--only-batch", action="store_true",
+                    help="run only the multi-RHS workload (24^3x48, 12 RHS over 3 kappa), print its "
+                         "report, then exit (for hardware-counter runs)")
     ap.add_argument("--hmc", type=int, default=0, metavar="NTRAJ",
                     help="run NTRAJ HMC trajectories of the trajectory workload (two Hasenbusch "
                          "pseudofermions, Omelyan, tau=1, 20 outer steps, from a start thermalised "
@@ -472,6 +475,14 @@ def main():
         stream = torch.cuda.Stream(device=dev)
         torch.cuda.set_stream(stream)
         print(json.dumps({"hmc": hmc(args, torch, dev, stream)}), flush=True)
+        return 0
+
+    if args.only_batch:
+        dev = torch.device("cuda:0")
+        torch.cuda.set_device(dev)
+        stream = torch.cuda.Stream(device=dev)
+        torch.cuda.set_stream(stream)
+        print(json.dumps({"batched": batched(args, torch, dev, stream)}), flush=True)
         return 0
 
     L = tuple(int(x) for x in args.lattice.split(","))

@@ -114,6 +114,12 @@ def lib():
         L.hb_cg_d.argtypes = [_vp, ctypes.c_int, ctypes.c_double, ctypes.c_double, _vp, _vp,
                               ctypes.c_float, ctypes.c_int, ctypes.c_int, _ip, _dp]
         L.hb_set_link_recon.argtypes = [_vp, ctypes.c_int]
+        L.hb_dsf_dir.argtypes = [_vp, ctypes.c_int, ctypes.c_int, _dp, _dp, _dp, ctypes.c_double, _dp,
+                                 ctypes.POINTER(ctypes.c_int), ctypes.c_int]
+        L.hb_clover_mult_field.argtypes = [_vp, ctypes.c_int, _dp, _dp, _dp]
+        L.hb_clover_dsf.argtypes = [_vp, ctypes.c_int, _dp, ctypes.c_int, _dp, _dp, ctypes.c_double]
+        L.hb_clover_dsd.argtypes = [_vp, ctypes.c_int, _dp, ctypes.c_int, ctypes.c_double]
+        L.hb_clover_action_field.argtypes = [_vp, _dp, _dp]
         L.hb_cg_d_noalloc.argtypes = [_vp, ctypes.c_int, ctypes.c_double, ctypes.c_double, _vp, _vp,
                                       ctypes.c_float, ctypes.c_int, ctypes.c_int, _ip, _dp]
         L.hb_spinor_to_device.argtypes = [_vp, _vp, _dp]
@@ -471,6 +477,45 @@ class Lattice:
         _check(lib().hb_cg_d(self._h, OP[op], kappa, rho, self._s(x), self._s(b), ctypes.c_float(rest),
                              maxiter, log, ctypes.byref(it), ctypes.byref(tr)))
         return it.value, tr.value
+
+    # ---- entry points of the LIBD / LIBCLOVER plugins (host arrays) ----------
+    def dsf_dir(self, mu, forward, p, b, a, s, u, nn):
+        """dsf_{x,y,z,t}{f,b} (d/DSFxyzt.F90:77-117): p (volh, 8) of one (parity,
+        mu) accumulates in place; b, a (n_tot, 3, 4, 2) spinors with any halo
+        sites; u (volh, 3, 3, 2) links of (parity, mu); nn 1-based int32"""
+        assert p.flags["C_CONTIGUOUS"] and p.dtype == np.float64 and p.shape == (self.volh, 8)
+        b = np.ascontiguousarray(b, dtype=np.float64)
+        a = np.ascontiguousarray(a, dtype=np.float64)
+        assert a.shape == b.shape and a.shape[0] >= self.volh
+        nn = np.ascontiguousarray(nn, dtype=np.int32)
+        assert nn.shape == (self.volh,)
+        _check(lib().hb_dsf_dir(self._h, mu, int(forward), _p(p), _p(b), _p(a), s,
+                                _p(np.ascontiguousarray(u, dtype=np.float64)),
+                                nn.ctypes.data_as(ctypes.POINTER(ctypes.c_int)), a.shape[0]))
+
+    def clover_mult_field(self, mode, clov_parity, inp):
+        """clover_mult_a (mode 0) / clover_mult_b (mode 1) with one parity of a
+        clover field (volh, 2, 18, 2)"""
+        out = self.spinor()
+        _check(lib().hb_clover_mult_field(self._h, mode, _p(np.ascontiguousarray(clov_parity)), _p(out),
+                                          _p(np.ascontiguousarray(inp))))
+        return out
+
+    def clover_dsf(self, eo, p, b, a, s):
+        """clover_dsf (clover/clover_d.F90:78-106) on the loaded physical links;
+        p (4, 2, volh, 8) accumulates in place"""
+        _check(lib().hb_clover_dsf(self._h, eo, _p(p), p.shape[2], _p(np.ascontiguousarray(b)),
+                                   _p(np.ascontiguousarray(a)), s))
+
+    def clover_dsd(self, eo, p, s):
+        """clover_dsd (clover/clover_d.F90:40-75) with the loaded LDL^+ field"""
+        _check(lib().hb_clover_dsd(self._h, eo, _p(p), p.shape[2], s))
+
+    def clover_action_field(self, b_parity):
+        """clover_action (clover/clover_action.F90:29-62) of one parity (volh, 2, 18, 2)"""
+        s = ctypes.c_double(0.0)
+        _check(lib().hb_clover_action_field(self._h, _p(np.ascontiguousarray(b_parity)), ctypes.byref(s)))
+        return s.value
 
     def set_link_recon(self, recon):
         """links the FP64 CG iterates with: 12 (rows 0 and 1, row 2 rebuilt; the

@@ -20,7 +20,7 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
 
 SOURCES = ["hb_context.cpp", "hb_fields.hip", "hb_dslash.hip", "hb_blas.hip", "hb_cg.hip",
-           "hb_gauge.hip", "hb_mixed.hip", "hb_multi.hip", "hb_noalloc.hip",
+           "hb_gauge.hip", "hb_mixed.hip", "hb_multi.hip", "hb_noalloc.hip", "hb_plugin.hip",
            "hb_force.hip", "hb_hmc.hip", "hb_conf.cpp"]
 
 FLAGS = [

@@ -20,6 +20,18 @@ __global__ __launch_bounds__(kBlock) void k_clover_mult(SpinorPtr<S> out, Spinor
   store_spinor<S>(out, i, vh, y);
 }
 
+// out = A in (mode 0) or the LDL^+ solve (mode 1) with parity 0 of the FP64
+// clover field C ([2][36][vh] device layout)
+int clover_mult_parity_f64(Context *c, int mode, const double2 *C, double2 *out, const double2 *in) {
+  SpinorPtr<SF64> o{out, nullptr}, s{const_cast<double2 *>(in), nullptr};
+  if (mode == 1)
+    k_clover_mult<SF64, 1><<<nblocks(c->g.vh), kBlock, 0, c->stream>>>(o, s, C, nullptr, 0, c->g.vh);
+  else
+    k_clover_mult<SF64, 0><<<nblocks(c->g.vh), kBlock, 0, c->stream>>>(o, s, C, nullptr, 0, c->g.vh);
+  HB_HIP(hipGetLastError());
+  return HB_OK;
+}
+
 int clover_mult_f64(Context *c, int which, int eo, double2 *out, const double2 *in) {
   SpinorPtr<SF64> o{out, nullptr}, s{const_cast<double2 *>(in), nullptr};
   if (which == HB_CLOVER_B)

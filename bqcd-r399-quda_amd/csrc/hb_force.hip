@@ -512,6 +512,42 @@ int dsd_dev(Context *c, double *P, double step, double csw_kappa) {
   return HB_OK;
 }
 
+// clover_dsf (clover/clover_d.F90:78-106) of spinors b, a of parity eo with
+// scale s, on the loaded (physical) links; P accumulates
+int clover_dsf_dev(Context *c, double *P, int eo, const double2 *b, const double2 *a, double s) {
+  if (s == 0.0) return HB_OK;
+  const int vh = c->g.vh;
+  int rc = 0;
+  double2 *W = static_cast<double2 *>(scratch(c, SLOT_FORCE2, sizeof(double2) * 54 * (size_t)vh, &rc)); HB_TRY(rc);
+  k_w_bsa<<<nblocks(vh), kBlock, 0, c->stream>>>(W, b, a, vh);
+  k_clover_force<<<dim3(nblocks(vh, 64), 8), 64, 0, c->stream>>>(P, c->gauge, c->g, W, eo, s, nullptr, 0, 0.0);
+  HB_HIP(hipGetLastError());
+  return HB_OK;
+}
+
+// clover_dsd (clover/clover_d.F90:40-75) with the loaded LDL^+ field of parity eo
+int clover_dsd_dev(Context *c, double *P, int eo, double s) {
+  if (s == 0.0) return HB_OK;
+  HB_CHECK(c->have_clover[HB_CLOVER_B], HB_E_STATE, "clover_dsd: clover B field not present");
+  const int vh = c->g.vh;
+  int rc = 0;
+  double2 *W = static_cast<double2 *>(scratch(c, SLOT_FORCE2, sizeof(double2) * 54 * (size_t)vh, &rc)); HB_TRY(rc);
+  k_w_ts<<<nblocks(vh, 128), 128, 0, c->stream>>>(W, c->clov[HB_CLOVER_B], eo, vh);
+  k_clover_force<<<dim3(nblocks(vh, 64), 8), 64, 0, c->stream>>>(P, c->gauge, c->g, W, eo, s, nullptr, 0, 0.0);
+  HB_HIP(hipGetLastError());
+  return HB_OK;
+}
+
+// clover_action (clover/clover_action.F90:29-62) of parity eo of the LDL^+
+// field Cb (device layout [2][36][vh])
+int clover_action_parity_dev(Context *c, const double2 *Cb, int eo, double *res_d) {
+  const int vh = c->g.vh;
+  k_clover_action<<<nblocks(vh), kBlock, 0, c->stream>>>(Cb, eo, vh, c->red);
+  k_finalize<1><<<1, kFinThreads, 0, c->stream>>>(c->red, nblocks(vh), nullptr, FinTwice{res_d});
+  HB_HIP(hipGetLastError());
+  return HB_OK;
+}
+
 int dsg_dev(Context *c, double *P, double step, double beta) {
   if (beta == 0.0) return HB_OK;
   k_gauge_force<<<dim3(nblocks(c->g.vh, 128), 8), 128, 0, c->stream>>>(P, c->gauge, c->g, -step * beta / 3.0);

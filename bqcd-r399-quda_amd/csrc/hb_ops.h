@@ -56,13 +56,21 @@ enum {
   SLOT_HMC = SLOT_BACKUP + kGaugeBackups,           // HMC fields (hb_hmc.hip), 24 slots
   SLOT_HMC_MV = SLOT_HMC + 24,                      // kMaxMre products M v_i
   SLOT_HMC_MRE = SLOT_HMC_MV + kMaxMre,             // 2 bases of kMaxMre vectors
-  SLOT_END = SLOT_HMC_MRE + 2 * kMaxMre
+  SLOT_PLUGIN = SLOT_HMC_MRE + 2 * kMaxMre,         // staging of the plugin entry points (hb_plugin.hip), 6 slots
+  SLOT_END = SLOT_PLUGIN + 6
 };
 static_assert(SLOT_SINGLE_END <= SLOT_MULTI, "single-RHS slots overlap the multi-RHS range");
 static_assert(SLOT_MULTI_LAST < SLOT_MULTI_CLOVER_B, "multi-RHS fields overlap the clover scratch");
 static_assert(SLOT_MULTI_CLOVER_B < SLOT_MULTI_CLOVER, "clover scratch overlaps the per-kappa fields");
 static_assert(SLOT_MULTI_CLOVER + 2 * kMaxKappaGroups <= SLOT_BACKUP, "per-kappa clover overlaps the backups");
 static_assert(SLOT_BACKUP + kGaugeBackups <= SLOT_HMC, "gauge backups overlap the HMC fields");
+
+// one-parity clover and force pieces behind the plugin entry points (hb_force.hip)
+int clover_dsf_dev(Context *c, double *P, int eo, const double2 *b, const double2 *a, double s);
+int clover_dsd_dev(Context *c, double *P, int eo, double s);
+int clover_action_parity_dev(Context *c, const double2 *Cb, int eo, double *res_d);
+int clover_mult_parity_f64(Context *c, int mode, const double2 *C, double2 *out, const double2 *in);
+double *mom_scratch(Context *c, int *rc);
 
 bool use_clover(const Context *c);
 int check_ready(const Context *c);

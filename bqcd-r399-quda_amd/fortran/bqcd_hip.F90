@@ -216,6 +216,54 @@ module bqcd_hip
        real(c_double), value :: step, csw_kappa
      end function
 
+     integer(c_int) function hb_dsf_dir(ctx, mu, forward, p, b, a, s, u, nn, n_tot) &
+          bind(C, name="hb_dsf_dir")
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value    :: ctx
+       integer(c_int), value :: mu, forward
+       type(*)               :: p(*)
+       type(*), intent(in)   :: b(*), a(*)
+       real(c_double), value :: s
+       type(*), intent(in)   :: u(*)
+       integer(c_int), intent(in) :: nn(*)
+       integer(c_int), value :: n_tot
+     end function
+
+     integer(c_int) function hb_clover_mult_field(ctx, mode, clov, out, in) bind(C, name="hb_clover_mult_field")
+       import :: c_int, c_ptr
+       type(c_ptr), value    :: ctx
+       integer(c_int), value :: mode
+       type(*), intent(in)   :: clov(*)
+       type(*)               :: out(*)
+       type(*), intent(in)   :: in(*)
+     end function
+
+     integer(c_int) function hb_clover_dsf(ctx, eo, p, site_stride, b, a, s) bind(C, name="hb_clover_dsf")
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value    :: ctx
+       integer(c_int), value :: eo
+       type(*)               :: p(*)
+       integer(c_int), value :: site_stride
+       type(*), intent(in)   :: b(*), a(*)
+       real(c_double), value :: s
+     end function
+
+     integer(c_int) function hb_clover_dsd(ctx, eo, p, site_stride, s) bind(C, name="hb_clover_dsd")
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value    :: ctx
+       integer(c_int), value :: eo
+       type(*)               :: p(*)
+       integer(c_int), value :: site_stride
+       real(c_double), value :: s
+     end function
+
+     integer(c_int) function hb_clover_action_field(ctx, b, s_det) bind(C, name="hb_clover_action_field")
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value          :: ctx
+       type(*), intent(in)         :: b(*)
+       real(c_double), intent(out) :: s_det
+     end function
+
      integer(c_int) function hb_dsg(ctx, p, site_stride, step, beta) bind(C, name="hb_dsg")
        import :: c_int, c_ptr, c_double
        type(c_ptr), value    :: ctx

@@ -232,6 +232,34 @@ int hb_last_timing(hb_context *ctx, double *solve_ms, int *n_matvec);
  * in the reference */
 int hb_time_dslash_lp(hb_context *ctx, int precision, int reps, const double *in_d, double *ms);
 
+/* ---- entry points of the link-time plugins LIBD (d/Makefile:113-129) and
+ *      LIBCLOVER (clover/Makefile:44-71); host arrays in BQCD layouts, every
+ *      call uploads its operands and downloads its result.  d / d_dag are
+ *      hb_load_gauge + hb_dslash, clover_init is hb_load_gauge +
+ *      hb_build_clover + hb_save_clover. --------------------------------- */
+/* dsf_{x,y,z,t}{f,b} (d/DSFxyzt.F90:77-117): hopping-force piece of
+ * direction mu (0..3) forward or backward.  p: real(8, volh) of one (parity,
+ * mu), accumulates; b, a: spinors complex(4,3,n_tot) including any halo
+ * sites; u: complex(3,3,volh) links of (parity, mu); nn(volh): 1-based
+ * neighbour indices into 1..n_tot (checked on the host). */
+int hb_dsf_dir(hb_context *ctx, int mu, int forward, double *p, const double *b, const double *a,
+               double s, const double *u, const int *nn, int n_tot);
+/* clover_mult_a (mode 0: out = A in) and clover_mult_b (mode 1: out = the
+ * LDL^+ solve) with one parity of a clover field, type_clover_a/b(2, volh)
+ * (clover/clover_mult_a.F90:30-96, clover_mult_b.F90:30-97); clover_mult_ao
+ * is mode 0 with out == in */
+int hb_clover_mult_field(hb_context *ctx, int mode, const double *clov, double *out, const double *in);
+/* clover_dsf(eo, p, b, a, s, u) (clover/clover_d.F90:78-106) on the loaded
+ * links, which must be physical; p real(8, site_stride, 0:1, 4) accumulates */
+int hb_clover_dsf(hb_context *ctx, int eo, double *p, int site_stride, const double *b, const double *a,
+                  double s);
+/* clover_dsd(eo, p, b, s, u) (clover/clover_d.F90:40-75) with the loaded
+ * LDL^+ field (hb_load_clover) and links */
+int hb_clover_dsd(hb_context *ctx, int eo, double *p, int site_stride, double s);
+/* clover_action(b) (clover/clover_action.F90:29-62) of one parity of an LDL^+
+ * field, type_clover_b(2, volh) */
+int hb_clover_action_field(hb_context *ctx, const double *b, double *s_det);
+
 /* ---- HMC trajectory: replaces hmc() (hmc.F90:28-180) with the integrator
  *      of hmc_leap_frog.F90:28-79 / hmc_integrator.F90:28-103 ------------- */
 enum { HB_INT_LEAPFROG = 0,   /* hmc_leap_frog.F90: multi-time-scale leapfrog */

@@ -7,6 +7,7 @@ the host side falls back to the oracle when the library is absent."""
 import ctypes
 import os
 import re
+import subprocess
 
 import pytest
 
@@ -221,3 +222,26 @@ def test_multi_block_limit_keeps_buffers_below_4gib():
             nr = lim(v, p)
             assert 12 * v * nr * (8 if p == "f32" else 4) < 2 ** 32, (k, p)
     assert lim(0, "f16") == 0 and lim(vh["24x48"], "f64") == 0
+
+
+PLUGIN_ROUTINES = ["d", "d_dag", "version_of_d", "dsf_xf", "dsf_xb", "dsf_yf", "dsf_yb", "dsf_zf", "dsf_zb",
+                   "dsf_tf", "dsf_tb", "clover_init", "clover_mult_a", "clover_mult_ao", "clover_mult_b",
+                   "clover_dsf", "clover_dsd", "clover_action", "allocate_clover_field_a",
+                   "allocate_clover_field_b"]
+
+
+def test_plugin_file_provides_every_libd_and_libclover_routine():
+    """fortran/hip_plugins.F90 defines each routine BQCD links from LIBD
+    (d/Makefile:113-129) and LIBCLOVER (clover/Makefile:44-71), as external
+    procedures with the reference's names; where the plugin-linked driver has
+    been built, its object file defines them"""
+    text = open(os.path.join(REPO, "bqcd-r399-quda_amd", "fortran", "hip_plugins.F90")).read()
+    defined = set(re.findall(r"^\s*(?:REAL\s+|integer\s+)?(?:function|subroutine)\s+(\w+)\s*\(", text,
+                             flags=re.M | re.I))
+    missing = [n for n in PLUGIN_ROUTINES if n not in defined]
+    assert not missing, missing
+    obj = os.path.join(REPO, "oracle", "_ref", "build_seq", "hip_plugins.o")
+    if os.path.exists(obj):
+        nm = subprocess.run(["nm", obj], capture_output=True, text=True).stdout
+        syms = {line.split()[-1] for line in nm.splitlines() if " T " in line}
+        assert not [n for n in PLUGIN_ROUTINES if n + "_" not in syms]
