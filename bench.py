@@ -60,7 +60,8 @@ def parse():
     ap.add_argument("--delta", type=float, default=0.1)
     ap.add_argument("--max-inner", type=int, default=1000)
     ap.add_argument("--dslash-reps", type=int, default=50)
-    ap.add_argument("--cpu-iters", type=int, default=20)
+    ap.add_argument("--cpu-iters", type=int, default=200,
+                    help="reference CG iterations timed on the host cores for each CPU baseline")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-sweep", action="store_true",
                     help="skip the precision comparison at the configuration's reliable-update "
@@ -118,9 +119,19 @@ def parse():
     ap.add_argument("--hmc-first", type=int, default=0,
                     help="index of the first measured trajectory (its draws and accept number): "
                          "a run split over several jobs continues where the last one stopped")
+    ap.add_argument("--hmc-restart", action="store_true",
+                    help="start every measured trajectory from the thermalised start (independent "
+                         "proposals with their own draws) instead of chaining them; a run split over "
+                         "several jobs (--hmc-first) then gives the same trajectories as one job")
     ap.add_argument("--hmc-runs", default="mixed16_batched,fp64",
                     help="solver runs of the trajectory workload, comma separated "
                          "(mixed16_batched, mixed32_batched, fp64)")
+    ap.add_argument("--reference-workload", choices=["smp", "mpp"], default=None,
+                    help="run the reference's own benchmark input (work/input.smp: 24^3x48, one "
+                         "trajectory of 10 leapfrog steps; work/input.mpp: 48^3x96, two trajectories "
+                         "of 20 steps; beta 5, kappa 0.13, csw 1.923, cold start, every CG stopped "
+                         "after 100 iterations) on the GPU, and for smp the reference hmc() on the "
+                         "host cores (unless --no-cpu); print the timings, then exit")
     ap.add_argument("--scan", action="store_true",
                     help="print plaquette and FP64 iteration counts per smearing sweep, then exit")
     return ap.parse_args()
@@ -167,13 +178,44 @@ def tile_links(us, Ls, L):
     return out
 
 
-def cpu_baseline(g, b_dev, kappa, csw, gpu_iters, cpu_iters):
-    """Times the reference Fortran CG (or the oracle) on the same field."""
+def cpu_model():
+    """the host CPU's model name and the number of logical CPUs this process may use"""
+    name = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    name = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return name, len(os.sched_getaffinity(0))
+
+
+def cpu_threads():
+    return int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or len(os.sched_getaffinity(0))
+
+
+def raise_stack_limit():
+    """child set-up for the reference's OpenMP programs: the main thread's stack
+    raised to the hard limit (at 24^3x48 with 16 threads the reference's
+    trajectory overflows the usual 8 MiB)"""
+    import resource
+
+    soft, hard = resource.getrlimit(resource.RLIMIT_STACK)
+    resource.setrlimit(resource.RLIMIT_STACK, (hard, hard))
+
+
+def cpu_baseline(g, b_dev, kappa, csw, gpu_iters, cpu_iters, what="the GPU FP64 solve"):
+    """Times the reference Fortran CG (or the oracle) on the same field: cpu_iters
+    iterations, scaled to gpu_iters."""
     import numpy as np
 
     ref = os.path.join(REPO, "oracle", "_ref", "bqcd_golden_omp")
-    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or os.cpu_count()
+    threads = cpu_threads()
+    model, ncpu = cpu_model()
     L = g.L
+    lat_s = "x".join(map(str, L))
     if os.path.exists(ref):
         with tempfile.TemporaryDirectory(dir="/tmp") as d:
             u = g.save_gauge(flipped=False)
@@ -186,7 +228,8 @@ def cpu_baseline(g, b_dev, kappa, csw, gpu_iters, cpu_iters):
                 *L, kappa, csw, cpu_iters, d, d, cpu_iters)
             env = dict(os.environ, OMP_NUM_THREADS=str(threads))
             t0 = time.time()
-            p = subprocess.run([ref], input=ctl.encode(), capture_output=True, env=env, timeout=600)
+            p = subprocess.run([ref], input=ctl.encode(), capture_output=True, env=env, timeout=900,
+                               preexec_fn=raise_stack_limit)
             wall = time.time() - t0
             out = p.stdout.decode()
             if p.returncode != 0:
@@ -200,10 +243,14 @@ def cpu_baseline(g, b_dev, kappa, csw, gpu_iters, cpu_iters):
             "unit": "s",
             "cores": threads,
             "kind": "reference",
+            "cpu_model": model,
+            "cpus_available": ncpu,
+            "ms_per_iteration": 1e3 * per_iter,
             "sample": "reference BQCD Fortran cg() (libd21, single_pe, OpenMP, amdflang -O2) "
-                      "on the same 32^3x64 field: %d iterations in %.2f s (%.1f ms/iter, "
-                      "process wall %.1f s incl. clover_init and I/O), scaled to the %d "
-                      "iterations of the GPU FP64 solve" % (it, cg_s, 1e3 * per_iter, wall, gpu_iters),
+                      "on the same %s field, %d OpenMP threads on %s: %d iterations in %.2f s "
+                      "(%.1f ms/iter, process wall %.1f s incl. clover_init and I/O), scaled to "
+                      "the %d iterations of %s" % (lat_s, threads, model, it, cg_s, 1e3 * per_iter,
+                                                  wall, gpu_iters, what),
         }
     # oracle port (single thread)
     sys.path.insert(0, os.path.join(REPO, "tests"))
@@ -218,8 +265,10 @@ def cpu_baseline(g, b_dev, kappa, csw, gpu_iters, cpu_iters):
     lat.cg(op, 2, eta, 1e-30, cpu_iters)
     dt = time.time() - t0
     return {"value": dt / cpu_iters * gpu_iters, "unit": "s", "cores": 1, "kind": "port",
-            "sample": "oracle C restatement cg(), %d iterations in %.2f s, scaled to %d iterations"
-                      % (cpu_iters, dt, gpu_iters)}
+            "cpu_model": model, "cpus_available": ncpu, "ms_per_iteration": 1e3 * dt / cpu_iters,
+            "sample": "oracle C restatement cg() on the same %s field, 1 thread on %s: %d iterations "
+                      "in %.2f s, scaled to the %d iterations of %s"
+                      % (lat_s, model, cpu_iters, dt, gpu_iters, what)}
 
 
 def batched(args, torch, dev, stream):
@@ -293,6 +342,15 @@ def batched(args, torch, dev, stream):
         "max_true_residual_rel": max(trs_blk[k] / (rests[k] * 1e20) for k in range(len(kappas))),
         "max_solution_rel_error_vs_fp64": err,
     }
+    if not args.no_cpu:
+        # the reference CG's time per iteration on this field (the operator costs
+        # the same for every kappa), times the iterations of the 12 FP64 solves
+        try:
+            res["cpu_baseline"] = cpu_baseline(g, bs[0], kappas[0], args.csw, sum(its_f64), args.cpu_iters,
+                                               "the 12 sequential FP64 GPU solves")
+        except Exception as exc:  # reported, never silently replaced
+            log("batched cpu baseline failed:", exc)
+            res["cpu_baseline"] = None
     log("batched:", json.dumps(res))
     g.close()
     return res
@@ -374,7 +432,7 @@ def hmc(args, torch, dev, stream):
         make_start(gs)
         g.load_gauge(tile_links(gs.save_gauge(), Ls, L))
         gs.close()
-        log("hmc start: the links repeated %s times: plaquette %.6f"
+        log("hmc start: the links repeated %s times: plaquette %.12f"
             % ("x".join(str(n // s) for n, s in zip(L, Ls)), g.plaquette()))
     else:
         make_start(g)
@@ -396,6 +454,8 @@ def hmc(args, torch, dev, stream):
                                   mre_vectors=args.hmc_mre, batch=batch, verbose=True)
         trajs = []
         for k in range(args.hmc):
+            if args.hmc_restart and k > 0:
+                g.gauge_restore(1)
             seed = args.seed + 100 * (args.hmc_first + k)
             g.gaussian_d(p, seed + 1, 1.0)
             g.gaussian_d(eta1, seed + 2, 0.5)
@@ -425,6 +485,7 @@ def hmc(args, torch, dev, stream):
                          % (args.hmc_steps, args.hmc_mscale), "mre_vectors": args.hmc_mre,
            "model": args.hmc_model,
            "trajectories": args.hmc, "first_trajectory": args.hmc_first,
+           "each_from_start": args.hmc_restart,
            "thermalisation": "%s%s%d force-accepted trajectories of length 0.5 in %d steps%s"
                              % ("on %s: " % args.hmc_tile.replace(",", "x") if args.hmc_tile else "",
                                 "%d pure-gauge MD trajectories at beta %.3f, then " % (args.hmc_quench, args.hmc_quench_beta)
@@ -436,6 +497,90 @@ def hmc(args, torch, dev, stream):
         res["speedup_vs_fp64"] = (runs["fp64"]["s_per_trajectory"]
                                   / runs["mixed16_batched"]["s_per_trajectory"])
     g.close()
+    return res
+
+
+def reference_workload(args, torch, dev, stream):
+    """The reference's own benchmark inputs, work/input.smp and work/input.mpp:
+    beta 5.0, kappa 0.13, csw 1.923076923076923, model A (no Hasenbusch),
+    leapfrog, cold start, solver_rest 1e-99, solver_maxiter 100 and
+    solver_ignore_no_convergence 2, so that every CG runs exactly 100
+    iterations.  smp: 24^3x48, one force-accepted trajectory of 10 steps of
+    0.02 (11 CG calls); mpp: 48^3x96, two trajectories of 20 steps of 0.01
+    (21 CG calls each), the first force-accepted.  The FP64 drop-in path runs
+    them; momenta and noise come from the device generator (the reference
+    draws its own: the work is fixed by the input, the draws move only the end
+    point).  Time per trajectory as the library measures it.  For smp the
+    reference hmc() itself (oracle/_ref/bqcd_traj_omp, built from the
+    reference sources with OpenMP) runs the same input on the host cores."""
+    import numpy as np
+    import bqcd_hip
+
+    cases = {"smp": ((24, 24, 24, 48), 10, 0.02, 1), "mpp": ((48, 48, 48, 96), 20, 0.01, 2)}
+    L, ntau, tau, ntraj = cases[args.reference_workload]
+    beta, kappa, csw = 5.0, 0.13, 1.923076923076923
+    g = bqcd_hip.Lattice(L)
+    g.set_stream(stream)
+    vh = g.volh
+
+    def cold():
+        u = np.zeros((4, 2, vh, 3, 3, 2))
+        for k in range(3):
+            u[:, :, :, k, k, 0] = 1.0
+        return u
+
+    g.load_gauge(cold())
+    prm = bqcd_hip.hmc_params(beta=beta, kappa=kappa, csw=csw, rho=0.0, tau=tau, ntau=ntau, m_scale=1,
+                              model="A", integrator="leapfrog", rest=1e-99, maxiter=100, log=2,
+                              mre_vectors=0)
+    p = torch.empty((4, 2, 8, vh), dtype=torch.float64, device=dev)
+    eta = torch.empty((12, vh, 2), dtype=torch.float64, device=dev)
+    trajs = []
+    for k in range(ntraj):
+        g.gaussian_d(p, args.seed + 31 + 10 * k, 1.0)
+        g.gaussian_d(eta, args.seed + 32 + 10 * k, 0.5)
+        out = g.hmc_trajectory_d(prm, p, eta, accept_random=-1.0 if k == 0 else 0.5)
+        log("reference workload %s trajectory %d: %d CG calls, %d iterations (max %d), plaquette %.10f, "
+            "%.2f s" % (args.reference_workload, k, out["cg_ncall"], out["cg_niter_tot"], out["cg_niter_max"],
+                        out["plaquette"], out["seconds"]))
+        trajs.append({"seconds": out["seconds"], "cg_calls": out["cg_ncall"], "cg_iterations": out["cg_niter_tot"],
+                      "cg_iterations_max": out["cg_niter_max"], "plaquette": out["plaquette"],
+                      "accepted": out["accepted"]})
+    del p, eta
+    g.close()
+    res = {"workload": "work/input.%s of the reference: %s, beta %g, kappa %g, csw %.6f, model A, leapfrog "
+                       "%d x %g, cold start, CGs of exactly 100 iterations"
+                       % (args.reference_workload, "x".join(map(str, L)), beta, kappa, csw, ntau, tau),
+           "gpu": {"path": "FP64 drop-in (hb_hmc_trajectory_d)", "trajectories": trajs,
+                   "seconds_total": sum(t["seconds"] for t in trajs)}}
+    if args.reference_workload == "smp" and not args.no_cpu:
+        ref = os.path.join(REPO, "oracle", "_ref", "bqcd_traj_omp")
+        if not os.path.exists(ref):
+            res["cpu"] = None
+            log("reference workload: %s not built, no CPU run" % ref)
+        else:
+            threads = cpu_threads()
+            model, ncpu = cpu_model()
+            with tempfile.TemporaryDirectory(dir="/tmp") as d:
+                cold().astype("<f8").tofile(os.path.join(d, "u.bin"))
+                ctl = "%d %d %d %d\n%.17g %.17g %.17g 0\n%.17g %d 1 A\n1e-99 100 0 2\n0\n%s\n%s\n" % (
+                    *L, beta, kappa, csw, tau, ntau, d, d)
+                env = dict(os.environ, OMP_NUM_THREADS=str(threads))
+                t0 = time.time()
+                r = subprocess.run([ref], input=ctl.encode(), capture_output=True, env=env, cwd=d, timeout=1000,
+                                   preexec_fn=raise_stack_limit)
+                wall = time.time() - t0
+                if r.returncode != 0:
+                    raise RuntimeError("reference trajectory failed (exit status %d): %s %s"
+                                       % (r.returncode, r.stdout.decode()[-500:], r.stderr.decode()[-500:]))
+                info = dict(line.split() for line in open(os.path.join(d, "info.txt")))
+            res["cpu"] = {"kind": "reference", "program": "reference hmc() (traj driver, libd21, single_pe, "
+                                                          "OpenMP, amdflang -O2), its own draws",
+                          "cores": threads, "cpu_model": model, "cpus_available": ncpu,
+                          "seconds": wall, "cg_calls": int(float(info["cg_ncall"])),
+                          "cg_iterations": int(float(info["cg_niter_tot"])),
+                          "note": "process wall time, including reading the 24^3x48 start links"}
+            log("reference workload smp on the host: %.1f s, %d threads (%s)" % (wall, threads, model))
     return res
 
 
@@ -467,6 +612,14 @@ def main():
         return 2
     import torch
     import bqcd_hip
+
+    if args.reference_workload:
+        dev = torch.device("cuda:0")
+        torch.cuda.set_device(dev)
+        stream = torch.cuda.Stream(device=dev)
+        torch.cuda.set_stream(stream)
+        print(json.dumps({"reference_workload": reference_workload(args, torch, dev, stream)}), flush=True)
+        return 0
 
     if args.hmc:
         hmc_lattices(args)  # argument errors before any GPU work
