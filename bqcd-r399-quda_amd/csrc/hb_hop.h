@@ -30,7 +30,8 @@ __device__ __forceinline__ void hop(typename S::V (&acc)[12], const typename G::
   V h[6], w[6];
   if constexpr (S::kNorm || link_raw<G>()) {
     // 16-bit words are used as stored: the spinor's site factor, the link's
-    // 1/32767 and the boundary sign go into one multiply of the half spinor
+    // 1/32767 and the boundary sign make one factor f of the hop, applied by
+    // the multiply-adds that spread U h into acc
     A f;
     if constexpr (MU == 3) {
       V p[6];
@@ -48,11 +49,9 @@ __device__ __forceinline__ void hop(typename S::V (&acc)[12], const typename G::
     V u[9];
     if (FWD) f *= load_link_f<G>(u, U, MU, e, i, vh);
     else f *= load_link_f<G>(u, U, MU, 1 - e, j, vh);
-#pragma unroll
-    for (int k = 0; k < 6; k++) h[k] = cscale<V>(f, h[k]);
     if (FWD) su3_mul_half(w, u, h);
     else su3_adj_mul_half(w, u, h);
-    reconstruct<MU, SG>(acc, w);
+    reconstruct<MU, SG, true>(acc, w, f);
   } else {
     if constexpr (MU == 3) {
       V p[6];

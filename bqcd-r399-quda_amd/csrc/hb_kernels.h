@@ -503,31 +503,44 @@ __device__ __forceinline__ void project(V (&h)[6], const V (&p)[12]) {
   }
 }
 
-template <int MU, int SG, class V>
-__device__ __forceinline__ void reconstruct(V (&acc)[12], const V (&w)[6]) {
+// acc += s w (s real)
+template <class V, class A>
+__device__ __forceinline__ V caxpy(V acc, A s, V w) {
+  return cmk<V>(s * w.x + acc.x, s * w.y + acc.y);
+}
+
+// acc += (1 + s gamma_mu) spread of the half spinor w; with SCALED the half
+// spinor is first multiplied by f (folded into the adds as multiply-adds)
+template <int MU, int SG, bool SCALED = false, class V>
+__device__ __forceinline__ void reconstruct(V (&acc)[12], const V (&w)[6], decltype(V{}.x) f = 1) {
+  // a + f v (plus) or a - f v
+  auto add = [f](V a, V v, bool plus) {
+    if constexpr (SCALED) return caxpy(a, plus ? f : -f, v);
+    else return plus ? cadd(a, v) : csub(a, v);
+  };
 #pragma unroll
   for (int c = 0; c < 3; c++) {
     V w0 = w[2 * c], w1 = w[2 * c + 1];
     if constexpr (MU == 3) {
       if constexpr (SG < 0) {  // (1 - gamma_t): spins 3,4
-        acc[6 + c] = cadd(acc[6 + c], w0);
-        acc[9 + c] = cadd(acc[9 + c], w1);
+        acc[6 + c] = add(acc[6 + c], w0, true);
+        acc[9 + c] = add(acc[9 + c], w1, true);
       } else {
-        acc[c] = cadd(acc[c], w0);
-        acc[3 + c] = cadd(acc[3 + c], w1);
+        acc[c] = add(acc[c], w0, true);
+        acc[3 + c] = add(acc[3 + c], w1, true);
       }
     } else {
-      acc[c] = cadd(acc[c], w0);
-      acc[3 + c] = cadd(acc[3 + c], w1);
+      acc[c] = add(acc[c], w0, true);
+      acc[3 + c] = add(acc[3 + c], w1, true);
       if constexpr (MU == 0) {  // lo0 = -s i w1, lo1 = -s i w0
-        acc[6 + c] = SG > 0 ? csub(acc[6 + c], ci(w1)) : cadd(acc[6 + c], ci(w1));
-        acc[9 + c] = SG > 0 ? csub(acc[9 + c], ci(w0)) : cadd(acc[9 + c], ci(w0));
+        acc[6 + c] = add(acc[6 + c], ci(w1), SG < 0);
+        acc[9 + c] = add(acc[9 + c], ci(w0), SG < 0);
       } else if constexpr (MU == 1) {  // lo0 = -s w1, lo1 = s w0
-        acc[6 + c] = SG > 0 ? csub(acc[6 + c], w1) : cadd(acc[6 + c], w1);
-        acc[9 + c] = SG > 0 ? cadd(acc[9 + c], w0) : csub(acc[9 + c], w0);
+        acc[6 + c] = add(acc[6 + c], w1, SG < 0);
+        acc[9 + c] = add(acc[9 + c], w0, SG > 0);
       } else {  // MU == 2: lo0 = -s i w0, lo1 = s i w1
-        acc[6 + c] = SG > 0 ? csub(acc[6 + c], ci(w0)) : cadd(acc[6 + c], ci(w0));
-        acc[9 + c] = SG > 0 ? cadd(acc[9 + c], ci(w1)) : csub(acc[9 + c], ci(w1));
+        acc[6 + c] = add(acc[6 + c], ci(w0), SG < 0);
+        acc[9 + c] = add(acc[9 + c], ci(w1), SG > 0);
       }
     }
   }

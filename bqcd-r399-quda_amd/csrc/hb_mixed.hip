@@ -18,6 +18,7 @@ namespace hb {
 
 struct MixState {
   double rtr, rtr_new, pap, beta, maxr;
+  double rp_re, rp_im;  // <r, p> / |r|^2 after a reliable update
   double rest, delta;
   int iter, since, max_inner, maxiter;
   int flag;      // 0 iterate, 1 reliable update due, 2 converged, 3 maxiter reached
@@ -205,13 +206,52 @@ struct FinMixTrue {
   }
 };
 
-// p = r_lo + beta p after a reliable update (beta = 0 at the start: p = r)
+// Re-orthogonalisation of the kept search direction at a reliable update:
+// <r_lo, p> (complex) over the new residual, so that the restart can take out
+// the component of p along r that the replaced residual no longer cancels
+// (the reliable-update CG of Clark et al., "Solving lattice QCD systems of
+// equations using mixed precision solvers on GPUs", 2010)
+#ifndef HB_MIX_REORTH
+#define HB_MIX_REORTH 1
+#endif
+template <class S, class R>
+__global__ __launch_bounds__(kBlock) void k_mix_rdotp(SpinorPtr<S> p, SpinorPtr<R> r, int vh, const MixState *st,
+                                                      double *partials) {
+  using V = typename S::V;
+  if (st->flag) return;
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  double re = 0.0, im = 0.0;
+  if (i < vh) {
+    V pv[12], rv[12];
+    load_spinor<S>(pv, p, i, vh);
+    load_spinor<R>(rv, r, i, vh);
+#pragma unroll
+    for (int k = 0; k < 12; k++) {  // conj(r) p
+      re += (double)rv[k].x * pv[k].x + (double)rv[k].y * pv[k].y;
+      im += (double)rv[k].x * pv[k].y - (double)rv[k].y * pv[k].x;
+    }
+  }
+  double vals[2] = {re, im};
+  block_partials<2>(vals, partials);
+}
+
+struct FinMixRdotp {
+  MixState *st;
+  __device__ void operator()(const double (&t)[2]) const {
+    st->rp_re = t[0] / st->rtr;
+    st->rp_im = t[1] / st->rtr;
+  }
+};
+
+// p = r_lo + beta (p - rp r_lo) after a reliable update (rp = <r, p>/|r|^2, or
+// 0; beta = 0 at the start: p = r)
 template <class S, class R>
 __global__ __launch_bounds__(kBlock) void k_mix_prestart(SpinorPtr<S> p, SpinorPtr<R> r, int vh,
                                                          const MixState *st) {
   using V = typename S::V;
   if (st->flag) return;
   const float beta = (float)st->beta;
+  const float cr = (float)st->rp_re, ci = (float)st->rp_im;
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= vh) return;
   V pv[12], rv[12];
@@ -224,8 +264,10 @@ __global__ __launch_bounds__(kBlock) void k_mix_prestart(SpinorPtr<S> p, SpinorP
   }
 #pragma unroll
   for (int k = 0; k < 12; k++) {
-    pv[k].x = rv[k].x + beta * pv[k].x;
-    pv[k].y = rv[k].y + beta * pv[k].y;
+    const float ux = pv[k].x - (cr * rv[k].x - ci * rv[k].y);
+    const float uy = pv[k].y - (cr * rv[k].y + ci * rv[k].x);
+    pv[k].x = rv[k].x + beta * ux;
+    pv[k].y = rv[k].y + beta * uy;
   }
   store_spinor<S>(p, i, vh, pv);
 }
@@ -373,6 +415,11 @@ int cg_mixed_impl(Context *c, int op, double kappa, double rho, double2 *x, cons
     k_mix_true_residual<R><<<nblocks(vh), kBlock, 0, c->stream>>>(ax, b, r, vh, c->red);
     k_finalize<1><<<1, kFinThreads, 0, c->stream>>>(c->red, nblocks(vh), nullptr, FinMixTrue{st, first});
     HB_HIP(hipGetLastError());
+    if (!first && HB_MIX_REORTH) {
+      k_mix_rdotp<S, R><<<nblocks(vh), kBlock, 0, c->stream>>>(p, r, vh, st, c->red);
+      k_finalize<2><<<1, kFinThreads, 0, c->stream>>>(c->red, nblocks(vh), &st->flag, FinMixRdotp{st});
+      HB_HIP(hipGetLastError());
+    }
     k_mix_prestart<S, R><<<nblocks(vh), kBlock, 0, c->stream>>>(p, r, vh, st);
     HB_HIP(hipGetLastError());
     nmv++;
