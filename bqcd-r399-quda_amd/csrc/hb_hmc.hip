@@ -349,8 +349,10 @@ struct Traj {
     return dsf_dev(c, P, step, prm.kappa, ck, a2, b2);
   }
 
-  // model B with batching: both pseudofermion systems in one multi-RHS solve
-  int dsf12_batched(double step, bool calc) {
+  // both pseudofermion systems in one multi-RHS solve (model B: both on the
+  // outer steps; model C: at the points where an inner and an outer kick meet),
+  // S_f1's force with step1, S_f2's with step2
+  int dsf12_batched(double step1, double step2, bool calc) {
     HB_TRY(mre_get(0, prm.rho, a, phi1));
     HB_TRY(dsf2_src());
     double kap[2] = {prm.kappa, prm.kappa}, rh[2] = {prm.rho, 0.0};
@@ -363,9 +365,10 @@ struct Traj {
                     prm.precision, 2, it, tr));
     count(it[0]);
     count(it[1]);
-    HB_TRY(dsf1_tail(step, calc));
-    return dsf2_tail(step, calc);
+    HB_TRY(dsf1_tail(step1, calc));
+    return dsf2_tail(step2, calc);
   }
+  bool batched() const { return prm.batch && prm.precision != HB_PREC_F64; }
 
   int dsd(double step) { return clover ? dsd_dev(c, P, step, ck) : HB_OK; }
 
@@ -377,8 +380,8 @@ struct Traj {
         HB_TRY(dsf1(step, calc));
         return dsd(step);
       case 'B':
-        if (prm.batch && prm.precision != HB_PREC_F64) {
-          HB_TRY(dsf12_batched(step, calc));
+        if (batched()) {
+          HB_TRY(dsf12_batched(step, step, calc));
         } else {
           HB_TRY(dsf1(step, calc));
           HB_TRY(dsf2(step, calc));
@@ -398,6 +401,21 @@ struct Traj {
     return HB_OK;
   }
   int q(double step) { return hmc_u_dev(c, P, step, clover ? ck : 0.0); }
+
+  // an inner and an outer kick at the same links (momentum kicks commute):
+  // under model C with the batched solver the inner S_f1 system and the outer
+  // S_f2 system are solved together as one 2-RHS block
+  int p_both(double s_uv, double s_ir, bool calc) {
+    if (prm.model == 'C' && batched()) {
+      n_uv++;
+      n_ir++;
+      HB_TRY(dsg_dev(c, P, s_uv, prm.beta));
+      HB_TRY(dsf12_batched(s_uv, s_ir, calc));
+      return dsd(s_uv);
+    }
+    HB_TRY(p_uv(s_uv, calc));
+    return p_ir(s_ir, calc);
+  }
 
   // hmc_leap_frog.F90:28-79
   int leapfrog() {
@@ -426,31 +444,30 @@ struct Traj {
 
   // Omelyan (second-order minimum norm) on both scales: per outer step
   // P_ir(l h) U_uv(h/2) P_ir((1-2l) h) U_uv(h/2) P_ir(l h), with U_uv(h) made
-  // of m_scale inner steps P_uv(l e) Q(e/2) P_uv((1-2l) e) Q(e/2) P_uv(l e);
-  // the momentum kicks at step boundaries are merged
+  // of m_scale inner steps P_uv(l e) Q(e/2) P_uv((1-2l) e) Q(e/2) P_uv(l e).
+  // Kicks between two link updates act on the same links and are merged:
+  // P_uv(2 l e) between inner steps, and where one inner sequence ends and the
+  // next begins the inner P_uv(2 l e) and the outer kick together (p_both)
   int omelyan() {
     const double l = prm.lambda > 0.0 ? prm.lambda : 0.1931833275037836;
     const int m = prm.m_scale;
     const double h = prm.tau, e = 0.5 * h / m;
-    HB_TRY(p_ir(l * h, false));
-    double pending_uv = l * e;  // first inner kick
+    HB_TRY(p_both(l * e, l * h, false));
     for (int it = 1; it <= prm.ntau; it++) {
       for (int half = 0; half < 2; half++) {
         for (int is = 1; is <= m; is++) {
-          HB_TRY(p_uv(pending_uv, false));
           HB_TRY(q(0.5 * e));
           HB_TRY(p_uv((1.0 - 2.0 * l) * e, false));
           HB_TRY(q(0.5 * e));
-          pending_uv = (is < m) ? 2.0 * l * e : l * e;
-          if (is < m) continue;
-          const bool last = it == prm.ntau && half == 1;
-          HB_TRY(p_uv(pending_uv, last));
-          if (half == 0) {
-            HB_TRY(p_ir((1.0 - 2.0 * l) * h, false));
+          if (is < m) {
+            HB_TRY(p_uv(2.0 * l * e, false));
+          } else if (half == 0) {
+            HB_TRY(p_both(2.0 * l * e, (1.0 - 2.0 * l) * h, false));
+          } else if (it == prm.ntau) {
+            HB_TRY(p_both(l * e, l * h, true));
           } else {
-            HB_TRY(p_ir(last ? l * h : 2.0 * l * h, last));
+            HB_TRY(p_both(2.0 * l * e, 2.0 * l * h, false));
           }
-          pending_uv = l * e;
         }
       }
       progress(it);

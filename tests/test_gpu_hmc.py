@@ -179,16 +179,24 @@ def test_delta_h_scales_as_step_squared(hip, integrator):
     g.close()
 
 
-def test_batched_model_b_matches_sequential(hip):
+@pytest.mark.parametrize("model", ["B", "C"])
+def test_batched_matches_sequential(hip, model):
+    """Model B solves both pseudofermion systems of an outer step as one 2-RHS
+    block; model C does so where an inner and an outer kick of the Omelyan
+    integrator meet (the S_f1 and S_f2 systems at the same links).  Against
+    the same trajectory with single-RHS solves.  Omelyan merges the kicks
+    between two link updates: 4 ntau m + 1 inner and 2 ntau + 1 outer force
+    evaluations."""
     L = (4, 4, 4, 6)
     lat = ol.Lattice(L)
     u0 = lat.gauge_exp_random(6, 0.2)
     draws = lat.hmc_draws(78)
     g = hip.Lattice(L)
     res = {}
+    ntau, m = 4, 2
     for batch in (False, True):
-        prm = hip.hmc_params(beta=5.29, kappa=0.13, csw=1.9, rho=0.1, tau=0.05, ntau=4, m_scale=2,
-                             model="B", integrator="omelyan", rest=1e-20, maxiter=5000,
+        prm = hip.hmc_params(beta=5.29, kappa=0.13, csw=1.9, rho=0.1, tau=0.05, ntau=ntau, m_scale=m,
+                             model=model, integrator="omelyan", rest=1e-20, maxiter=5000,
                              precision="f32", batch=batch)
         g.load_gauge(u0)
         res[batch] = (g.hmc_trajectory(prm, *draws), g.save_gauge())
@@ -196,6 +204,9 @@ def test_batched_model_b_matches_sequential(hip):
     print(a[0], b[0], flush=True)
     assert abs(a[0]["h_new"] - a[0]["h_old"] - (b[0]["h_new"] - b[0]["h_old"])) < 1e-7
     assert np.max(np.abs(a[1] - b[1])) < 1e-9
+    for out in (a[0], b[0]):
+        assert (out["n_force_uv"], out["n_force_ir"]) == (4 * ntau * m + 1, 2 * ntau + 1)
+        assert out["cg_ncall"] == a[0]["cg_ncall"]
     g.close()
 
 
