@@ -18,6 +18,8 @@ namespace hb {
 
 struct MixState {
   double rtr, rtr_new, pap, beta, maxr;
+  double alpha;  // of the last iteration, for the x_lo update
+  int xpend;     // the last launched iteration ran: its x_lo update is due
   double rp_re, rp_im;  // <r, p> / |r|^2 after a reliable update
   double rest, delta;
   int iter, since, max_inner, maxiter;
@@ -77,10 +79,14 @@ __global__ void k_cvt_spinor(SpinorPtr<S> dst, const double2 *__restrict__ src, 
 }
 
 // ---- inner iteration kernels -----------------------------------------------------
-// alpha = rtr / pAp; x_lo += alpha p; r -= alpha q; rtr_new = |r|^2; decide
+// The solution increment since the last reliable update, x_lo, is FP32 (in 16
+// bits its rounding, amplified by the operator's high modes, stalled the FP32
+// inner solver).  The update x_lo += alpha p happens in the p update, which
+// reads p anyway.
+
+// alpha = rtr / pAp; r -= alpha q; rtr_new = |r|^2
 template <class S, class R>
-__global__ __launch_bounds__(kBlock) void k_mix_update(float2 *__restrict__ xlo, SpinorPtr<R> r,
-                                                       SpinorPtr<S> p, SpinorPtr<S> q, int vh,
+__global__ __launch_bounds__(kBlock) void k_mix_update(SpinorPtr<R> r, SpinorPtr<S> q, int vh,
                                                        const MixState *st, double *partials) {
   using V = typename S::V;
   if (st->flag) return;
@@ -88,16 +94,11 @@ __global__ __launch_bounds__(kBlock) void k_mix_update(float2 *__restrict__ xlo,
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   double s = 0.0;
   if (i < vh) {
-    V pv[12], qv[12], rv[12];
-    load_spinor<S>(pv, p, i, vh);
+    V qv[12], rv[12];
     load_spinor<S>(qv, q, i, vh);
     load_spinor<R>(rv, r, i, vh);
 #pragma unroll
     for (int k = 0; k < 12; k++) {
-      float2 xv = xlo[(size_t)k * vh + i];
-      xv.x += alpha * pv[k].x;
-      xv.y += alpha * pv[k].y;
-      xlo[(size_t)k * vh + i] = xv;
       rv[k].x -= alpha * qv[k].x;
       rv[k].y -= alpha * qv[k].y;
       s += (double)rv[k].x * rv[k].x + (double)rv[k].y * rv[k].y;
@@ -109,9 +110,17 @@ __global__ __launch_bounds__(kBlock) void k_mix_update(float2 *__restrict__ xlo,
 }
 
 // iterated |r|^2 -> reliable-update decision or beta (cg.F90:109-115)
+// (launched unconditionally: after a skipped update it only clears xpend, so
+// that the p update of a skipped iteration adds nothing to x_lo)
 struct FinMixUpdate {
   MixState *st;
   __device__ void operator()(const double (&t)[1]) const {
+    if (st->flag) {
+      st->xpend = 0;
+      return;
+    }
+    st->alpha = st->rtr / st->pap;  // the alpha of k_mix_update
+    st->xpend = 1;
     const double rn = t[0];
     st->iter += 1;
     st->since += 1;
@@ -128,24 +137,37 @@ struct FinMixUpdate {
   }
 };
 
-// p = r + beta p
+// x_lo += alpha p (every iteration that ran), then p = r + beta p (unless a
+// reliable update or the end is due)
 template <class S, class R>
-__global__ __launch_bounds__(kBlock) void k_mix_pupdate(SpinorPtr<S> p, SpinorPtr<R> r, int vh,
-                                                        const MixState *st) {
+__global__ __launch_bounds__(kBlock) void k_mix_pupdate(SpinorPtr<S> p, SpinorPtr<R> r, float2 *__restrict__ xlo,
+                                                        int vh, const MixState *st) {
   using V = typename S::V;
-  if (st->flag) return;
+  if (!st->xpend) return;
+  const float alpha = (float)st->alpha;
   const float beta = (float)st->beta;
+  const bool next = st->flag == 0;
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= vh) return;
-  V pv[12], rv[12];
+  V pv[12];
   load_spinor<S>(pv, p, i, vh);
-  load_spinor<R>(rv, r, i, vh);
 #pragma unroll
   for (int k = 0; k < 12; k++) {
-    pv[k].x = rv[k].x + beta * pv[k].x;
-    pv[k].y = rv[k].y + beta * pv[k].y;
+    float2 xv = xlo[(size_t)k * vh + i];
+    xv.x += alpha * pv[k].x;
+    xv.y += alpha * pv[k].y;
+    xlo[(size_t)k * vh + i] = xv;
   }
-  store_spinor<S>(p, i, vh, pv);
+  if (next) {
+    V rv[12];
+    load_spinor<R>(rv, r, i, vh);
+#pragma unroll
+    for (int k = 0; k < 12; k++) {
+      pv[k].x = rv[k].x + beta * pv[k].x;
+      pv[k].y = rv[k].y + beta * pv[k].y;
+    }
+    store_spinor<S>(p, i, vh, pv);
+  }
 }
 
 // ---- reliable update kernels ------------------------------------------------------
@@ -404,11 +426,10 @@ int cg_mixed_impl(Context *c, int op, double kappa, double rho, double2 *x, cons
   HB_HIP(hipMemcpyAsync(st, hs, sizeof(MixState), hipMemcpyHostToDevice, c->stream));
   HB_TRY(zero_f64(c, reinterpret_cast<double2 *>(xlo), (size_t)6 * vh));
 
-  const int nvec = 12 * vh;
   int nmv = 0;
   auto reliable = [&](int first) -> int {
     if (!first) {
-      k_mix_accum<<<nblocks(std::min(nvec, 1 << 22)), kBlock, 0, c->stream>>>(x, xlo, (size_t)nvec);
+      k_mix_accum<<<nblocks(std::min(12 * vh, 1 << 22)), kBlock, 0, c->stream>>>(x, xlo, (size_t)12 * vh);
       HB_HIP(hipGetLastError());
     }
     HB_TRY(apply_f64(c, op, kappa, rho, ax, x, t64, y64, nullptr, nullptr));
@@ -432,10 +453,10 @@ int cg_mixed_impl(Context *c, int op, double kappa, double rho, double2 *x, cons
     for (int k = 0; k < chunk; k++) {
       HB_TRY((mtil_lp<S, G>(c, false, kappa, r_op, y, p, t, nullptr, SpinorPtr<S>{}, skip)));
       HB_TRY((mtil_lp<S, G>(c, true, kappa, r_op, q, y, t, &st->pap, p, skip)));
-      k_mix_update<S, R><<<nblocks(vh), kBlock, 0, c->stream>>>(xlo, r, p, q, vh, st, c->red);
-      k_finalize<1><<<1, kFinThreads, 0, c->stream>>>(c->red, nblocks(vh), &st->flag, FinMixUpdate{st});
+      k_mix_update<S, R><<<nblocks(vh), kBlock, 0, c->stream>>>(r, q, vh, st, c->red);
+      k_finalize<1><<<1, kFinThreads, 0, c->stream>>>(c->red, nblocks(vh), nullptr, FinMixUpdate{st});
       HB_HIP(hipGetLastError());
-      k_mix_pupdate<S, R><<<nblocks(vh), kBlock, 0, c->stream>>>(p, r, vh, st);
+      k_mix_pupdate<S, R><<<nblocks(vh), kBlock, 0, c->stream>>>(p, r, xlo, vh, st);
       HB_HIP(hipGetLastError());
     }
     HB_HIP(hipMemcpyAsync(hs, st, sizeof(MixState), hipMemcpyDeviceToHost, c->stream));

@@ -27,6 +27,8 @@ double *multi_partials(Context *c, int NR, int *rc);
 
 struct MultiState {
   double rtr[16], pap[16], beta[16], maxr[16], rest[16], res[16];
+  double alpha[16];  // of each RHS's last iteration, for its x_lo update
+  int xpend[16];     // that iteration ran: its x_lo update is due
   int flag[16];   // 0 iterate, 1 reliable update due, 2 converged, 3 maxiter, 4 unused lane
   int iter[16], since[16], nrel[16];
   double delta;
@@ -209,7 +211,12 @@ template <int NR> struct FinMultiUpdate {
   __device__ void operator()(const double (&t)[NR]) const {
 #pragma unroll
     for (int k = 0; k < NR; k++) {
-      if (st->flag[k] != 0) continue;
+      if (st->flag[k] != 0) {
+        st->xpend[k] = 0;  // the p update of a skipped iteration adds nothing to x_lo
+        continue;
+      }
+      st->alpha[k] = st->rtr[k] / st->pap[k];  // the alpha of k_multi_update
+      st->xpend[k] = 1;
       const double rn = t[k];
       st->iter[k] += 1;
       st->since[k] += 1;
@@ -227,9 +234,10 @@ template <int NR> struct FinMultiUpdate {
 };
 
 // vector kernels (thread per (site, RHS)) ------------------------------------------
+// As in the single-RHS mixed solver, the FP32 increments x_lo are updated in
+// the p update.
 template <class S, int NR>
-__global__ __launch_bounds__(kBlock) void k_multi_update(float2 *__restrict__ xlo, SpinorPtr<S> r,
-                                                         SpinorPtr<S> p, SpinorPtr<S> q, int vh,
+__global__ __launch_bounds__(kBlock) void k_multi_update(SpinorPtr<S> r, SpinorPtr<S> q, int vh,
                                                          const MultiState *st, double *partials) {
   using V = typename S::V;
   const int tid = blockIdx.x * blockDim.x + threadIdx.x;
@@ -240,17 +248,12 @@ __global__ __launch_bounds__(kBlock) void k_multi_update(float2 *__restrict__ xl
   for (int k = 0; k < NR; k++) vals[k] = 0.0;
   if (site < vh && st->flag[k0] == 0) {
     const float alpha = (float)(st->rtr[k0] / st->pap[k0]);
-    V pv[12], qv[12], rv[12];
-    load_spinor<S>(pv, p, tid, vhe);
+    V qv[12], rv[12];
     load_spinor<S>(qv, q, tid, vhe);
     load_spinor<S>(rv, r, tid, vhe);
     double s = 0.0;
 #pragma unroll
     for (int k = 0; k < 12; k++) {
-      float2 xv = xlo[(size_t)k * vhe + tid];
-      xv.x += alpha * pv[k].x;
-      xv.y += alpha * pv[k].y;
-      xlo[(size_t)k * vhe + tid] = xv;
       rv[k].x -= alpha * qv[k].x;
       rv[k].y -= alpha * qv[k].y;
       s += (double)rv[k].x * rv[k].x + (double)rv[k].y * rv[k].y;
@@ -263,17 +266,28 @@ __global__ __launch_bounds__(kBlock) void k_multi_update(float2 *__restrict__ xl
   block_partials<NR>(vals, partials);
 }
 
+// x_lo += alpha p for every RHS whose iteration ran, then p = r + beta p for
+// those that go on
 template <class S, int NR>
-__global__ __launch_bounds__(kBlock) void k_multi_pupdate(SpinorPtr<S> p, SpinorPtr<S> r, int vh,
-                                                          const MultiState *st) {
+__global__ __launch_bounds__(kBlock) void k_multi_pupdate(SpinorPtr<S> p, SpinorPtr<S> r, float2 *__restrict__ xlo,
+                                                          int vh, const MultiState *st) {
   using V = typename S::V;
   const int tid = blockIdx.x * blockDim.x + threadIdx.x;
   const int site = tid / NR, k0 = tid - site * NR;
-  if (site >= vh || st->flag[k0] != 0) return;
+  if (site >= vh || !st->xpend[k0]) return;
   const int vhe = vh * NR;
-  const float beta = (float)st->beta[k0];
-  V pv[12], rv[12];
+  const float alpha = (float)st->alpha[k0], beta = (float)st->beta[k0];
+  V pv[12];
   load_spinor<S>(pv, p, tid, vhe);
+#pragma unroll
+  for (int k = 0; k < 12; k++) {
+    float2 xv = xlo[(size_t)k * vhe + tid];
+    xv.x += alpha * pv[k].x;
+    xv.y += alpha * pv[k].y;
+    xlo[(size_t)k * vhe + tid] = xv;
+  }
+  if (st->flag[k0] != 0) return;
+  V rv[12];
   load_spinor<S>(rv, r, tid, vhe);
 #pragma unroll
   for (int k = 0; k < 12; k++) {
@@ -620,9 +634,9 @@ int cg_multi_block(Context *c, const MultiPlan &plan, const MultiFields<S, G> &f
     HB_TRY((multi_apply<S, G, NR>(c, f, plan.ngroups, grp, kap, rh, q, p, t, y, st, plan.clover)));
     const int tile = c->multi_tile;
     const int grid = nblocks(vh * NR, tile);
-    k_multi_update<S, NR><<<grid, tile, 0, c->stream>>>(xlo, r, p, q, vh, st, part);
+    k_multi_update<S, NR><<<grid, tile, 0, c->stream>>>(r, q, vh, st, part);
     k_finalize<NR><<<1, kFinThreads, 0, c->stream>>>(part, grid, nullptr, FinMultiUpdate<NR>{st});
-    k_multi_pupdate<S, NR><<<grid, tile, 0, c->stream>>>(p, r, vh, st);
+    k_multi_pupdate<S, NR><<<grid, tile, 0, c->stream>>>(p, r, xlo, vh, st);
     HB_HIP(hipGetLastError());
     return HB_OK;
   };
