@@ -110,6 +110,87 @@ template <class S> constexpr int hops_per_fence() {
   return sizeof(typename S::ST) == 16 ? 1 : HB_HOPS_PER_FENCE_REDUCED;
 }
 
+// ---- 16-bit hops with the loads of the next hop in flight ------------------
+// A 16-bit hop is split into its loads (the stored words, no conversion) and
+// its arithmetic; the hopping matrix fetches hop k+1 before it computes hop k,
+// so that every hop's arithmetic overlaps the next hop's memory latency.
+#ifndef HB_HOP_PIPELINE
+#define HB_HOP_PIPELINE 1
+#endif
+
+struct HopWords16 {
+  uint32_t s[13];  // neighbour spinor words (a t hop uses s[0..5]) and its scale word s[12]
+  uint32_t l[6];   // link rows 0 and 1
+  float sgn;       // boundary sign of the hop
+};
+
+template <class S, class G, int MU, bool FWD, int SG, int NR, class View>
+__device__ __forceinline__ void hop_fetch16(HopWords16 &d, const typename G::ST *U, const View &in,
+                                            const Geom &g, int e, int i, const int (&xc)[4], int r) {
+  int c[4] = {xc[0], xc[1], xc[2], xc[3]};
+  d.sgn = 1.0f;
+  if (FWD) {
+    c[MU] += 1;
+    if (c[MU] == g.L[MU]) { c[MU] = 0; d.sgn = (float)g.bc[MU]; }
+  } else {
+    c[MU] -= 1;
+    if (c[MU] < 0) { c[MU] = g.L[MU] - 1; d.sgn = (float)g.bc[MU]; }
+  }
+  const int j = site_index(g, c[0], c[1], c[2], c[3]);
+  if constexpr (MU == 3) in.fetch_pair(d.s, j * NR + r, SG < 0 ? 2 : 0);
+  else in.template fetch_dir<MU>(d.s, j * NR + r);
+  if (FWD) fetch_link16(d.l, U, MU, e, i, g.vh);
+  else fetch_link16(d.l, U, MU, 1 - e, j, g.vh);
+}
+
+template <int MU, bool FWD, int SG>
+__device__ __forceinline__ void hop_finish16(float2 (&acc)[12], const HopWords16 &d) {
+  float2 h[6], w[6];
+  float f;
+  if constexpr (MU == 3) {
+#pragma unroll
+    for (int cc = 0; cc < 3; cc++) {
+      h[2 * cc] = SF16::raw(as_short2(d.s[cc]));
+      h[2 * cc + 1] = SF16::raw(as_short2(d.s[3 + cc]));
+    }
+    f = 2.0f * d.sgn * as_float(d.s[12]) * SF16::kInv;
+  } else {
+    float2 p[12];
+#pragma unroll
+    for (int k = 0; k < 12; k++) p[k] = SF16::raw(as_short2(d.s[k]));
+    project<MU, SG>(h, p);
+    f = d.sgn * as_float(d.s[12]) * SF16::kInv;
+  }
+  float2 u[9];
+  f *= decode_link16(u, d.l);
+  if (FWD) su3_mul_half(w, u, h);
+  else su3_adj_mul_half(w, u, h);
+  reconstruct<MU, SG, true>(acc, w, f);
+}
+
+template <class S, class G, bool DAG, int NR, class View>
+__device__ __forceinline__ void hop_all_pipelined16(float2 (&acc)[12], const typename G::ST *U,
+                                                    const View &in, const Geom &g, int e, int i,
+                                                    const int (&xc)[4], int r) {
+  constexpr int SF = DAG ? 1 : -1;
+  constexpr int SB = -SF;
+  HopWords16 a, b;
+#define HB_F(MU, FWD, SG, D) hop_fetch16<S, G, MU, FWD, SG, NR, View>(D, U, in, g, e, i, xc, r); \
+  __builtin_amdgcn_sched_barrier(0);
+#define HB_C(MU, FWD, SG, D) hop_finish16<MU, FWD, SG>(acc, D); HB_SCHED_FENCE();
+  HB_F(3, true, SF, a)
+  HB_F(3, false, SB, b) HB_C(3, true, SF, a)
+  HB_F(2, true, SF, a) HB_C(3, false, SB, b)
+  HB_F(2, false, SB, b) HB_C(2, true, SF, a)
+  HB_F(1, true, SF, a) HB_C(2, false, SB, b)
+  HB_F(1, false, SB, b) HB_C(1, true, SF, a)
+  HB_F(0, true, SF, a) HB_C(1, false, SB, b)
+  HB_F(0, false, SB, b) HB_C(0, true, SF, a)
+  HB_C(0, false, SB, b)
+#undef HB_F
+#undef HB_C
+}
+
 // all 8 hops of the hopping matrix into acc, t first (the reference's
 // direction order does not matter for the sum); the forward projector sign is
 // (1 - gamma) for D and (1 + gamma) for D^+
@@ -117,6 +198,10 @@ template <class S, class G, bool DAG, int NR = 1, class View = SpinorView<S>>
 __device__ __forceinline__ void hop_all(typename S::V (&acc)[12], const typename G::ST *U,
                                         const View &in, const Geom &g, int e, int i,
                                         const int (&xc)[4], int r = 0) {
+  if constexpr (HB_HOP_PIPELINE && std::is_same<S, SF16>::value && std::is_same<G, SF16>::value) {
+    hop_all_pipelined16<S, G, DAG, NR, View>(acc, U, in, g, e, i, xc, r);
+    return;
+  }
   constexpr int F = hops_per_fence<S>();
   constexpr int SF = DAG ? 1 : -1;
   constexpr int SB = -SF;

@@ -271,6 +271,23 @@ template <> struct SpinorView<SF16> {
   __device__ __forceinline__ float load_raw_dir(float2 (&v)[12], int j) const {
     return load_raw(v, j);
   }
+  // the same reads without the conversion, for a hop fetched ahead of its
+  // arithmetic: w[0..11] (or w[0..5] for a spin pair) and the scale in w[12]
+  template <int MU>
+  __device__ __forceinline__ void fetch_dir(uint32_t (&w)[13], int j) const { load_words(w, j); }
+  __device__ __forceinline__ void fetch_pair(uint32_t (&w)[13], int j, int s0) const {
+    const uint32_t vo = (uint32_t)j * 16u;
+    if (s0 == 0) {
+      const u32x4 a = bld_u32x4(r, vo, 0);
+      const u32x2 b = bld_u32x2(r, vo, plane);
+      w[0] = a[0]; w[1] = a[1]; w[2] = a[2]; w[3] = a[3]; w[4] = b[0]; w[5] = b[1];
+    } else {
+      const u32x2 a = bld_u32x2(r, vo + 8u, plane);
+      const u32x4 b = bld_u32x4(r, vo, 2 * plane);
+      w[0] = a[0]; w[1] = a[1]; w[2] = b[0]; w[3] = b[1]; w[4] = b[2]; w[5] = b[3];
+    }
+    w[12] = __builtin_amdgcn_raw_buffer_load_b32(rn, (uint32_t)j * 4u, 0, 0);
+  }
 };
 
 // 16-bit neighbour spinors shared through the wavefront: every lane holds the
@@ -288,6 +305,27 @@ struct WaveView16 {
   __device__ __forceinline__ float load_raw(float2 (&v)[12], int j) const { return g.load_raw(v, j); }
   __device__ __forceinline__ float load_pair_raw(float2 (&v)[6], int j, int s0) const {
     return g.load_pair_raw(v, j, s0);
+  }
+  __device__ __forceinline__ void fetch_pair(uint32_t (&x)[13], int j, int s0) const { g.fetch_pair(x, j, s0); }
+  // raw words of the neighbour j: from the lane that holds them, or loaded by
+  // the lanes whose neighbour is outside the wave (no conversion here, so the
+  // wait for those loads falls at the words' first use)
+  template <int MU>
+  __device__ __forceinline__ void fetch_dir(uint32_t (&x)[13], int j) const {
+    if constexpr (MU >= 2) {
+      g.load_words(x, j);
+    } else {
+      if (MU == 1 && !share_y) {
+        g.load_words(x, j);
+        return;
+      }
+      const int src = j - wave_base;
+      const bool inwave = (unsigned)src < 64u;
+      const int addr = (inwave ? src : 0) * 4;
+#pragma unroll
+      for (int k = 0; k < 13; k++) x[k] = (uint32_t)__builtin_amdgcn_ds_bpermute(addr, (int)w[k]);
+      if (!inwave) g.load_words(x, j);
+    }
   }
   template <int MU>
   __device__ __forceinline__ float load_raw_dir(float2 (&v)[12], int j) const {
@@ -418,6 +456,31 @@ __device__ __forceinline__ void load_link(typename S::V (&u)[9], const typename 
 // spinor, and rebuilds row 3 = conj(row0 x row1) / 32767 in the same units.
 // Other storages load the link as load_link and return 1.
 template <class G> constexpr bool link_raw() { return std::is_same<G, SF16>::value; }
+
+// the six stored words of a 16-bit link (rows 0 and 1), fetched ahead of use
+template <int POL = HB_REDUCED_OPERATOR_POLICY>
+__device__ __forceinline__ void fetch_link16(uint32_t (&w)[6], const SF16::ST *U, int mu, int eo, int i, int vh) {
+  rsrc_t r = make_rsrc(U + (size_t)((mu * 2 + eo) * 6) * vh, (uint32_t)vh * 24u);
+  const u32x4 a = bld_u32x4<POL>(r, (uint32_t)i * 16u, 0);
+  const u32x2 b = bld_u32x2<POL>(r, (uint32_t)i * 8u, (uint32_t)vh * 16u);
+  w[0] = a[0]; w[1] = a[1]; w[2] = a[2]; w[3] = a[3]; w[4] = b[0]; w[5] = b[1];
+}
+
+// the link of fetch_link16 as stored (integers as floats) with row 3 rebuilt
+// in the same units; returns the factor 1/32767
+__device__ __forceinline__ float decode_link16(float2 (&u)[9], const uint32_t (&w)[6]) {
+#pragma unroll
+  for (int k = 0; k < 6; k++) u[k] = SF16::raw(as_short2(w[k]));
+  auto cross = [](float2 a0, float2 b1, float2 a1, float2 b0) {  // conj(a0 b1 - a1 b0) / 32767
+    float2 t = cmul(a0, b1);
+    float2 s = cmul(a1, b0);
+    return make_float2((t.x - s.x) * SF16::kInv, -(t.y - s.y) * SF16::kInv);
+  };
+  u[6] = cross(u[1], u[5], u[2], u[4]);
+  u[7] = cross(u[2], u[3], u[0], u[5]);
+  u[8] = cross(u[0], u[4], u[1], u[3]);
+  return SF16::kInv;
+}
 
 template <class G, int POL = HB_REDUCED_OPERATOR_POLICY>
 __device__ __forceinline__ typename G::A load_link_f(typename G::V (&u)[9], const typename G::ST *U,
