@@ -162,10 +162,17 @@ __device__ __forceinline__ void hop_finish16(float2 (&acc)[12], const HopWords16
     f = d.sgn * as_float(d.s[12]) * SF16::kInv;
   }
   float2 u[9];
-  f *= decode_link16(u, d.l);
-  if (FWD) su3_mul_half(w, u, h);
-  else su3_adj_mul_half(w, u, h);
-  reconstruct<MU, SG, true>(acc, w, f);
+  decode_link16(u, d.l);
+  f *= SF16::kInv;
+  if (FWD) {
+    su3_mul_half(w, u, h);  // colour 3 of w from row 3 alone
+    reconstruct<MU, SG, true, true>(acc, w, f, f * SF16::kInv);
+  } else {
+    h[4] = cscale(SF16::kInv, h[4]);  // colour 3 of h meets row 3 alone
+    h[5] = cscale(SF16::kInv, h[5]);
+    su3_adj_mul_half(w, u, h);
+    reconstruct<MU, SG, true>(acc, w, f);
+  }
 }
 
 template <class S, class G, bool DAG, int NR, class View>

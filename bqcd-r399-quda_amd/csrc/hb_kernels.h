@@ -466,20 +466,21 @@ __device__ __forceinline__ void fetch_link16(uint32_t (&w)[6], const SF16::ST *U
   w[0] = a[0]; w[1] = a[1]; w[2] = a[2]; w[3] = a[3]; w[4] = b[0]; w[5] = b[1];
 }
 
-// the link of fetch_link16 as stored (integers as floats) with row 3 rebuilt
-// in the same units; returns the factor 1/32767
-__device__ __forceinline__ float decode_link16(float2 (&u)[9], const uint32_t (&w)[6]) {
+// the link of fetch_link16 as stored (integers as floats, value * 32767) with
+// row 3 = conj(row0 x row1) rebuilt from them and left in the units of the
+// product (value * 32767^2): its factor 1/32767 is applied once per hop by
+// the caller, to the two colour-3 outputs or inputs of the multiplication
+__device__ __forceinline__ void decode_link16(float2 (&u)[9], const uint32_t (&w)[6]) {
 #pragma unroll
   for (int k = 0; k < 6; k++) u[k] = SF16::raw(as_short2(w[k]));
-  auto cross = [](float2 a0, float2 b1, float2 a1, float2 b0) {  // conj(a0 b1 - a1 b0) / 32767
-    float2 t = cmul(a0, b1);
-    float2 s = cmul(a1, b0);
-    return make_float2((t.x - s.x) * SF16::kInv, -(t.y - s.y) * SF16::kInv);
+  auto cross = [](float2 a0, float2 b1, float2 a1, float2 b0) {  // conj(a0 b1 - a1 b0)
+    const float re = fmaf(a0.x, b1.x, fmaf(-a0.y, b1.y, fmaf(-a1.x, b0.x, a1.y * b0.y)));
+    const float mim = fmaf(a1.x, b0.y, fmaf(a1.y, b0.x, fmaf(-a0.x, b1.y, -(a0.y * b1.x))));
+    return make_float2(re, mim);
   };
   u[6] = cross(u[1], u[5], u[2], u[4]);
   u[7] = cross(u[2], u[3], u[0], u[5]);
   u[8] = cross(u[0], u[4], u[1], u[3]);
-  return SF16::kInv;
 }
 
 template <class G, int POL = HB_REDUCED_OPERATOR_POLICY>
@@ -574,15 +575,18 @@ __device__ __forceinline__ V caxpy(V acc, A s, V w) {
 
 // acc += (1 + s gamma_mu) spread of the half spinor w; with SCALED the half
 // spinor is first multiplied by f (folded into the adds as multiply-adds)
-template <int MU, int SG, bool SCALED = false, class V>
-__device__ __forceinline__ void reconstruct(V (&acc)[12], const V (&w)[6], decltype(V{}.x) f = 1) {
-  // a + f v (plus) or a - f v
-  auto add = [f](V a, V v, bool plus) {
-    if constexpr (SCALED) return caxpy(a, plus ? f : -f, v);
-    else return plus ? cadd(a, v) : csub(a, v);
-  };
+// (with F3, colour 3 by f3 in place of f)
+template <int MU, int SG, bool SCALED = false, bool F3 = false, class V>
+__device__ __forceinline__ void reconstruct(V (&acc)[12], const V (&w)[6], decltype(V{}.x) f = 1,
+                                            decltype(V{}.x) f3 = 1) {
 #pragma unroll
   for (int c = 0; c < 3; c++) {
+    const decltype(V{}.x) fc = (F3 && c == 2) ? f3 : f;
+    // a + fc v (plus) or a - fc v
+    auto add = [fc](V a, V v, bool plus) {
+      if constexpr (SCALED) return caxpy(a, plus ? fc : -fc, v);
+      else return plus ? cadd(a, v) : csub(a, v);
+    };
     V w0 = w[2 * c], w1 = w[2 * c + 1];
     if constexpr (MU == 3) {
       if constexpr (SG < 0) {  // (1 - gamma_t): spins 3,4
