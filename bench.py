@@ -804,8 +804,8 @@ def main():
                             "the A^-1 epilogue; 4 such launches per inner iteration)",
                   "bytes_per_launch": b16, "ms_per_launch": ms16,
                   "GFlops_hopping": DSLASH_FLOPS_PER_SITE * vh / (ms16 * 1e-3) / 1e9,
-                  "note": "no unit saturated at 32^3x64: VALU 62 %, texture addresser 56 %, "
-                          "HBM 5.5 TB/s (profiles/dslash_counters_32x32x32x64.md)"}
+                  "note": "no unit saturated at 32^3x64: VALU 60 %, texture addresser 71 %, "
+                          "HBM 5.7 TB/s (profiles/dslash_counters_32x32x32x64.md)"}
 
     # precision comparison (FP32 inner, then 16-bit storage) at the reliable-update
     # settings the precision configuration prescribes: update every 100 inner
