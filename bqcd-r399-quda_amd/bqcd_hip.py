@@ -156,6 +156,8 @@ def lib():
         L.hb_multi_tile.argtypes = [_vp]
         L.hb_multi_block_limit.argtypes = [ctypes.c_int, ctypes.c_int]
         L.hb_time_dslash_lp.argtypes = [_vp, ctypes.c_int, ctypes.c_int, _vp, _dp]
+        L.hb_apply_lp_d.argtypes = [_vp, ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_int,
+                                    _vp, _vp]
         L.hb_multi_set_tile.argtypes = [_vp, ctypes.c_int]
         L.hb_dsf.argtypes = [_vp, _dp, ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_double, _dp, _dp]
         L.hb_dsd.argtypes = [_vp, _dp, ctypes.c_int, ctypes.c_double, ctypes.c_double]
@@ -470,6 +472,11 @@ class Lattice:
 
     def apply_d(self, op, kappa, rho, out, inp):
         _check(lib().hb_apply_d(self._h, OP[op], kappa, rho, self._s(out), self._s(inp)))
+
+    def apply_lp_d(self, op, kappa, rho, out, inp, precision="f16"):
+        """out = op inp with the mixed solver's reduced-precision operator
+        (inp stored at that precision, the result decoded to FP64)"""
+        _check(lib().hb_apply_lp_d(self._h, OP[op], kappa, rho, PREC[precision], self._s(out), self._s(inp)))
 
     def cg_d(self, op, x, b, kappa, rho=0.0, rest=1e-20, maxiter=1000, log=0):
         it = ctypes.c_int(0)

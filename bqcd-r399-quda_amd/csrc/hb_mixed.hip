@@ -78,6 +78,17 @@ __global__ void k_cvt_spinor(SpinorPtr<S> dst, const double2 *__restrict__ src, 
   store_spinor<S>(dst, i, vh, v);
 }
 
+// reduced-precision field back to an FP64 spinor (the stored values, decoded)
+template <class S>
+__global__ void k_spinor_to_f64(double2 *__restrict__ dst, SpinorPtr<S> src, int vh) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= vh) return;
+  typename S::V v[12];
+  load_spinor<S>(v, src, i, vh);
+#pragma unroll
+  for (int k = 0; k < 12; k++) dst[(size_t)k * vh + i] = make_double2((double)v[k].x, (double)v[k].y);
+}
+
 // ---- inner iteration kernels -----------------------------------------------------
 // The solution increment since the last reliable update, x_lo, is FP32 (in 16
 // bits its rounding, amplified by the operator's high modes, stalled the FP32
@@ -539,6 +550,35 @@ int time_dslash_lp(Context *c, int reps, const double2 *in64, double *ms_out) {
   return HB_OK;
 }
 
+// the mixed solver's operator at reduced precision on an FP64 device spinor:
+// in is stored at precision S (links and clover at G), the operator kernels of
+// the inner iteration are applied (M~ or M~^+, with rho for W; M~^+ M~ or W^+ W
+// as the solver chains them), and the stored result is decoded to FP64
+template <class S, class G>
+int apply_lp(Context *c, int op, double kappa, double rho, double2 *out64, const double2 *in64) {
+  int rc = 0;
+  SpinorPtr<S> in = SpinorAlloc<S>::get(c, SLOT_LP0, SLOT_LPN, &rc); HB_TRY(rc);
+  SpinorPtr<S> out = SpinorAlloc<S>::get(c, SLOT_LP2, SLOT_LPN2, &rc); HB_TRY(rc);
+  SpinorPtr<S> t = SpinorAlloc<S>::get(c, SLOT_LP3, SLOT_LPN3, &rc); HB_TRY(rc);
+  SpinorPtr<S> y = SpinorAlloc<S>::get(c, SLOT_LP4, SLOT_LPN4, &rc); HB_TRY(rc);
+  HB_TRY(ensure_lowprec(c, G::kBytes == 8 ? 32 : 16));
+  const int vh = c->g.vh;
+  k_cvt_spinor<S><<<nblocks(vh), kBlock, 0, c->stream>>>(in, in64, vh);
+  HB_HIP(hipGetLastError());
+  const bool w = op == HB_OP_W || op == HB_OP_W_DAG || op == HB_OP_WDAGW;
+  const double r = w ? rho : 0.0;
+  if (op == HB_OP_MTDAGMT || op == HB_OP_WDAGW) {
+    HB_TRY((mtil_lp<S, G>(c, false, kappa, r, y, in, t, nullptr, SpinorPtr<S>{}, nullptr)));
+    HB_TRY((mtil_lp<S, G>(c, true, kappa, r, out, y, t, nullptr, SpinorPtr<S>{}, nullptr)));
+  } else {
+    const bool dag = op == HB_OP_MTIL_DAG || op == HB_OP_W_DAG;
+    HB_TRY((mtil_lp<S, G>(c, dag, kappa, r, out, in, t, nullptr, SpinorPtr<S>{}, nullptr)));
+  }
+  k_spinor_to_f64<S><<<nblocks(vh), kBlock, 0, c->stream>>>(out64, out, vh);
+  HB_HIP(hipGetLastError());
+  return HB_OK;
+}
+
 }  // namespace hb
 
 using namespace hb;
@@ -557,6 +597,22 @@ int hb_time_dslash_lp(hb_context *ctx, int precision, int reps, const double *in
   const double2 *in = reinterpret_cast<const double2 *>(in_d);
   return precision == HB_PREC_F16 ? time_dslash_lp<SF16>(c, reps, in, ms)
                                   : time_dslash_lp<SF32>(c, reps, in, ms);
+}
+
+int hb_apply_lp_d(hb_context *ctx, int op, double kappa, double rho, int precision, double *out_d,
+                  const double *in_d) {
+  Context *c = reinterpret_cast<Context *>(ctx);
+  HB_TRY(check_ready(c));
+  HB_CHECK(out_d && in_d && out_d != in_d, HB_E_ARG, "hb_apply_lp_d: bad pointers");
+  HB_CHECK(op >= HB_OP_MTIL && op <= HB_OP_WDAGW, HB_E_ARG, "hb_apply_lp_d: operator %d", op);
+  HB_HIP(hipSetDevice(c->device));
+  double2 *out = reinterpret_cast<double2 *>(out_d);
+  const double2 *in = reinterpret_cast<const double2 *>(in_d);
+  if (precision == HB_PREC_F32) return apply_lp<SF32, SF32>(c, op, kappa, rho, out, in);
+  if (precision == HB_PREC_F16) return apply_lp<SF16, SF16>(c, op, kappa, rho, out, in);
+  if (precision == HB_PREC_F16_GAUGE) return apply_lp<SF32, SF16>(c, op, kappa, rho, out, in);
+  set_error("hb_apply_lp_d: precision %d (use 32, 16 or 18)", precision);
+  return HB_E_ARG;
 }
 
 int hb_cg_mixed_d(hb_context *ctx, int op, double kappa, double rho, double *x_d, const double *b_d,

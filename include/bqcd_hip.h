@@ -231,6 +231,17 @@ int hb_last_timing(hb_context *ctx, double *solve_ms, int *n_matvec);
  * device spinor in_d (converted first); a benchmarking aid with no counterpart
  * in the reference */
 int hb_time_dslash_lp(hb_context *ctx, int precision, int reps, const double *in_d, double *ms);
+/* out_d = op in_d with the mixed solver's reduced-precision operator
+ * (precision HB_PREC_F32, HB_PREC_F16, or HB_PREC_F16_GAUGE: FP32 spinors with
+ * 16-bit links and clover): in_d is stored at that precision, the inner
+ * iteration's operator kernels are applied (op as hb_apply_d; M~^+ M~ and
+ * W^+ W as the solver chains M~ and M~^+), and the stored result is decoded to
+ * FP64.  Device spinors of one parity.  The operator at QUDA's sloppy
+ * precision (invert_quda.F90:313-316 sets cuda_prec_sloppy, to FP64 in the
+ * reference); it exposes the inner operator for verification against
+ * hb_apply_d */
+int hb_apply_lp_d(hb_context *ctx, int op, double kappa, double rho, int precision, double *out_d,
+                  const double *in_d);
 
 /* ---- entry points of the link-time plugins LIBD (d/Makefile:113-129) and
  *      LIBCLOVER (clover/Makefile:44-71); host arrays in BQCD layouts, every
